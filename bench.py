@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Collision-integral throughput on one MI355X.
+
+Workload (the headline configuration): hard-sphere molecules on a 32^3
+velocity grid cut to the sphere |v| < vmax = 4.8 (17,256 nodes), FP64, plain
+Korobov/projection method with 2^21 lattice nodes, one spatial cell whose f is
+a seeded mixture of 3 Maxwellians (n ~ U(0.2,1), T ~ U(0.3,1.5), |u| <= 1.2),
+explicit steps df/dt = Q(f,f) with dt = 0.02 and the conservation correction
+every step.  One step = one corrected Q(f,f) evaluation + the Euler update,
+replayed as a hipGraph.  Metric: Q(f,f) evaluations per second.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 5]
+
+Prints ONE JSON line.  The roofline object reports the dominant kernel's
+algorithmic bytes over its hipEvent-timed average duration (a separate
+instrumented pass after the timed region); cpu_baseline times the FP64 CPU
+oracle (oracle/, a scalar C restatement with OpenMP over lattice nodes) on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_GRID = 32
+VMAX = 4.8
+M_NODES = 1 << 21
+S0 = 0.25  # unit-diameter spheres: differential cross-section 1/4 per steradian
+DT = 0.02
+SEED = 2026
+
+
+def mixture(xi, rng, comps=3):
+    f = np.zeros(xi.shape[0])
+    for _ in range(comps):
+        n = rng.uniform(0.2, 1.0)
+        T = rng.uniform(0.3, 1.5)
+        while True:
+            u = rng.uniform(-1.2, 1.2, 3)
+            if u @ u <= 1.44:
+                break
+        f += n * (2 * np.pi * T) ** -1.5 * np.exp(-np.sum((xi - u) ** 2, axis=1) / (2 * T))
+    return f
+
+
+def moments(xi, f, d3v):
+    e = np.sum(xi * xi, axis=1)
+    return np.array([f.sum(), (f * xi[:, 0]).sum(), (f * xi[:, 1]).sum(), (f * xi[:, 2]).sum(), (f * e).sum()]) * d3v
+
+
+def entropy(f, d3v):
+    fp = np.maximum(f, 1e-300)
+    return float(np.sum(fp * np.log(fp)) * d3v)
+
+
+def cpu_baseline(xi, f0, n_pnt, d3v, budget_s=15.0):
+    """FP64 oracle (C, OpenMP over lattice nodes) on a bounded number of
+    evaluations of the same workload: stencil once (untimed), then corrected
+    Q(f,f) + Euler steps until budget_s has elapsed (at least 2)."""
+    import oracle as O
+    g = O.Grid((N_GRID,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+    a, _ = O.korobov_pick(M_NODES, SEED, 8)
+    st = O.korobov_stencil(g, M_NODES, a, S0)
+    f = f0.copy()[None]
+    O.korobov_collide(g, st, f)  # warm-up
+    times = []
+    t_all = time.perf_counter()
+    while len(times) < 2 or (time.perf_counter() - t_all < budget_s and len(times) < 50):
+        t0 = time.perf_counter()
+        lo, q = O.korobov_collide(g, st, f)
+        qc, _, _ = O.correct_loss(g.xi, lo, q, 1)
+        f = f + DT * qc
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / med, "unit": "Q(f,f) evals/s", "cores": threads, "kind": "port",
+            "sample": "%d corrected evaluations + Euler steps of the same 32^3 / 2^21-node cell "
+                      "(median of %d, stencil build untimed)" % (len(times), len(times))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the instrumented kernel-timing pass")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (cells shard with no exchange; "
+              "multi-GPU runs are not part of it): --gpus %d refused" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import gpu_determ_amd as G
+
+    ctx = G.Context((N_GRID,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+    t0 = time.perf_counter()
+    ctx.korobov_build(M_NODES, seed=SEED, s0=S0)
+    info = ctx.korobov_info()
+    build_s = time.perf_counter() - t0
+    rng = np.random.default_rng(SEED)
+    f0 = mixture(ctx.xi, rng)
+    fd = G.DeviceArray.from_host(f0[None])
+
+    flags = G.FLAG_CORRECT | G.FLAG_GRAPH
+    ctx.run_steps(G.METHOD_KOROBOV, fd, 1, DT, args.warmup, flags)
+    ctx.sync()
+    f_start = fd.get()[0]
+    t0 = time.perf_counter()
+    ctx.run_steps(G.METHOD_KOROBOV, fd, 1, DT, args.steps, flags)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    f_end = fd.get()[0]
+
+    m0 = moments(ctx.xi, f_start, ctx.d3v)
+    m1 = moments(ctx.xi, f_end, ctx.d3v)
+    cons = float(np.max(np.abs(m1 - m0) / np.maximum(np.abs(m0), np.abs(m0).max() * 1e-3)))
+    h0, h1 = entropy(f_start, ctx.d3v), entropy(f_end, ctx.d3v)
+
+    n = ctx.n_pnt
+    nn, ne = info["nodes"], info["entries"]
+    # algorithmic bytes per launch (cells = 1)
+    nparts = 256  # fused-kernel blocks for one cell
+    alg = {
+        # fused path: 20-byte entry records in CSR order + row pointers + f
+        # (once) + node centres in; Q, loss and moment partials out
+        "kor_fused": ne * 20 + 4 * (n + 1) + 8 * n + 24 * n + 16 * n + nparts * 160,
+        # partials + centres + f + loss + Q in; Q and f out
+        "kor_apply": nparts * 160 + 24 * n + 24 * n + 16 * n,
+        # two-phase path: stencil record (6 int32 + r + w) + node values out + f in
+        "kor_values": nn * (24 + 8 + 8) + nn * 32 + 8 * n,
+        # CSR entries + node values (each read once) + row pointers + loss, Q out
+        "kor_reduce": ne * 4 + nn * 32 + 4 * (n + 1) + 16 * n,
+        # single-block correction: f, loss, Q, xi in; Q and f out
+        "correct": 8 * n * 3 + 24 * n + 16 * n,
+    }
+    roof = None
+    kstats = {}
+    if not args.no_roofline:
+        ctx.kernel_stats_reset()
+        reps = max(10, min(50, args.steps))
+        ctx.run_steps(G.METHOD_KOROBOV, fd, 1, DT, reps, G.FLAG_CORRECT | G.FLAG_TIME)
+        ctx.sync()
+        for k in alg:
+            ms, cnt = ctx.kernel_stats(k)
+            if cnt:
+                kstats[k] = {"avg_us": 1e3 * ms / cnt, "launches": cnt,
+                             "alg_bytes": alg[k], "gbs": alg[k] / (ms / cnt * 1e-3) / 1e9}
+        dom = max(kstats, key=lambda k: kstats[k]["avg_us"])
+        ach = kstats[dom]["gbs"]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_us": round(kstats[dom]["avg_us"], 3), "alg_bytes": alg[dom],
+                "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                            for k, v in kstats.items()}}
+
+    if args.calib_copy_mib:
+        nb = args.calib_copy_mib << 20
+        a_ = G.DeviceArray(nb // 8)
+        b_ = G.DeviceArray(nb // 8)
+        G.check(G.lib().gdt_memcpy_dtod(b_.ptr, a_.ptr, nb))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ctx.xi, f0, n, ctx.d3v)
+
+    value = args.steps / elapsed
+    out = {
+        "metric": "Q(f,f) evals/s at 32^3 grid: plain FP64 vs mixed/surrogate @ rel-err 1e-4",
+        "value": round(value, 3),
+        "unit": "Q(f,f) evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 3-Maxwellian mixture)",
+        "config": {"workload": "plain FP64 Korobov/projection Q(f,f), full relaxation step", "grid": "32^3",
+                   "vmax": VMAX, "nodes": n, "korobov_M": M_NODES, "korobov_a": info["a"],
+                   "stencil_nodes": nn, "csr_entries": ne, "cells": 1, "dt": DT, "s0": S0,
+                   "correction": True, "graph": True, "stencil_build_s": round(build_s, 3)},
+        "checks": {"moment_drift_rel": cons, "entropy_start": h0, "entropy_end": h1,
+                   "min_f": float(f_end.min())},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

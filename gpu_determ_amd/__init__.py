@@ -1,0 +1,288 @@
+"""gpu_determ_amd: MI355X-native collision-integral library (host binding).
+
+The native code is libgdt.so (C ABI, include/gdt.h) with hand-written gfx950
+HIP kernels.  This module binds it with ctypes for tests, the benchmark and
+Python callers; it owns no compute path of its own.  Importing it does not
+touch the GPU; the library is loaded on first use and a missing build raises
+RuntimeError (there is no fallback).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgdt.so")
+
+LAYOUT_CELL_MAJOR = 0
+LAYOUT_NODE_MAJOR = 1
+TABLES_FP64 = 0
+TABLES_REFERENCE = 1
+METHOD_MATRIX = 0
+METHOD_KOROBOV = 1
+FLAG_CORRECT = 1
+FLAG_GRAPH = 2
+FLAG_TIME = 4
+FLAG_COURANT = 8
+FLAG_TWO_PHASE = 16
+
+# every symbol that include/gdt.h declares
+ABI_SYMBOLS = (
+    "gdt_last_error", "gdt_version", "gdt_device_count", "gdt_malloc", "gdt_free", "gdt_memcpy_htod",
+    "gdt_memcpy_dtoh", "gdt_memcpy_dtod", "gdt_memset", "gdt_create", "gdt_destroy", "gdt_synchronize",
+    "gdt_grid_info", "gdt_grid_nodes", "gdt_basis", "gdt_build_tables", "gdt_tables", "gdt_set_tables",
+    "gdt_korobov_build", "gdt_korobov_info", "gdt_korobov_export", "gdt_korobov_export_csr", "gdt_korobov_pick",
+    "gdt_collide_matrix", "gdt_collide_korobov", "gdt_correct", "gdt_evaluate", "gdt_relax", "gdt_run_steps",
+    "gdt_kernel_stats", "gdt_kernel_stats_reset", "gdt_transpose",
+)
+
+_lib = None
+
+
+class GdtError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("vmin", C.c_double * 3), ("vmax", C.c_double * 3), ("R", C.c_double)]
+
+
+def lib():
+    """Loads libgdt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("gpu_determ_amd/libgdt.so is not built: run `python -m gpu_determ_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.gdt_last_error.restype = C.c_char_p
+    L.gdt_korobov_pick.restype = C.c_uint32
+    L.gdt_korobov_pick.argtypes = [C.c_uint32, C.c_uint64, C.c_int, dp]
+    sigs = {
+        "gdt_version": [],
+        "gdt_device_count": [ip],
+        "gdt_malloc": [C.POINTER(vp), C.c_size_t],
+        "gdt_free": [vp],
+        "gdt_memcpy_htod": [vp, vp, C.c_size_t],
+        "gdt_memcpy_dtoh": [vp, vp, C.c_size_t],
+        "gdt_memcpy_dtod": [vp, vp, C.c_size_t],
+        "gdt_memset": [vp, C.c_int, C.c_size_t],
+        "gdt_create": [C.POINTER(vp), C.POINTER(GridDesc), C.c_int],
+        "gdt_destroy": [vp],
+        "gdt_synchronize": [vp],
+        "gdt_grid_info": [vp, ip, ip, dp, dp],
+        "gdt_grid_nodes": [vp, vp, vp],
+        "gdt_basis": [vp, vp],
+        "gdt_build_tables": [vp, C.c_int],
+        "gdt_tables": [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_long), C.POINTER(C.c_long)],
+        "gdt_set_tables": [vp, vp, vp],
+        "gdt_korobov_build": [vp, C.c_uint32, C.c_uint64, C.c_uint32, C.c_double],
+        "gdt_korobov_info": [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_long),
+                             C.POINTER(C.c_long), dp],
+        "gdt_korobov_export": [vp, vp, vp, vp, vp],
+        "gdt_korobov_export_csr": [vp, vp, vp],
+        "gdt_collide_matrix": [vp, vp, vp, vp, C.c_int, C.c_int],
+        "gdt_collide_korobov": [vp, vp, vp, vp, C.c_int, C.c_int],
+        "gdt_correct": [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int],
+        "gdt_evaluate": [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_uint],
+        "gdt_relax": [vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, C.c_int, C.c_uint],
+        "gdt_run_steps": [vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, C.c_uint],
+        "gdt_kernel_stats": [vp, C.c_char_p, dp, C.POINTER(C.c_long)],
+        "gdt_kernel_stats_reset": [vp],
+        "gdt_transpose": [vp, vp, vp, C.c_int, C.c_int],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise GdtError(lib().gdt_last_error().decode())
+    return rc
+
+
+def device_count():
+    n = C.c_int()
+    check(lib().gdt_device_count(C.byref(n)))
+    return n.value
+
+
+class DeviceArray:
+    """FP64 (or other dtype) buffer in device memory, owned by this object."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().gdt_malloc(C.byref(p), max(self.nbytes, 8)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        check(lib().gdt_memcpy_htod(self.ptr, a.ctypes.data, self.nbytes))
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().gdt_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().gdt_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().gdt_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A velocity grid on one GPU with its tables, stencil and scratch."""
+
+    def __init__(self, n, vmin, vmax, R=-1.0, device=0):
+        g = GridDesc()
+        for a in range(3):
+            g.n[a] = int(n[a])
+            g.vmin[a] = float(vmin[a])
+            g.vmax[a] = float(vmax[a])
+        g.R = float(R)
+        h = C.c_void_p()
+        check(lib().gdt_create(C.byref(h), C.byref(g), int(device)))
+        self.h = h.value
+        n_pnt, n_box = C.c_int(), C.c_int()
+        d = (C.c_double * 3)()
+        d3v = C.c_double()
+        check(lib().gdt_grid_info(self.h, C.byref(n_pnt), C.byref(n_box), d, C.byref(d3v)))
+        self.n = tuple(int(x) for x in n)
+        self.n_pnt, self.n_box = n_pnt.value, n_box.value
+        self.d = tuple(d)
+        self.d3v = d3v.value
+        self.cidx = np.empty((self.n_pnt, 3), np.int32)
+        self.xi = np.empty((self.n_pnt, 3))
+        check(lib().gdt_grid_nodes(self.h, self.cidx.ctypes.data, self.xi.ctypes.data))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gdt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().gdt_synchronize(self.h))
+
+    # -- tables ---------------------------------------------------------
+    def build_tables(self, mode=TABLES_FP64):
+        check(lib().gdt_build_tables(self.h, int(mode)))
+        self.sync()
+
+    def tables(self):
+        """Host copies of (b, a) and the nonzero count."""
+        b, a = C.c_void_p(), C.c_void_p()
+        n_a, nnz = C.c_long(), C.c_long()
+        check(lib().gdt_tables(self.h, C.byref(b), C.byref(a), C.byref(n_a), C.byref(nnz)))
+        hb = np.empty(self.n_box)
+        ha = np.empty(n_a.value)
+        check(lib().gdt_memcpy_dtoh(hb.ctypes.data, b.value, hb.nbytes))
+        check(lib().gdt_memcpy_dtoh(ha.ctypes.data, a.value, ha.nbytes))
+        return hb, ha, nnz.value
+
+    # -- Korobov ----------------------------------------------------------
+    def korobov_build(self, M, seed=0, a=0, s0=1.0):
+        check(lib().gdt_korobov_build(self.h, int(M), int(seed), int(a), float(s0)))
+
+    def korobov_info(self):
+        M, a = C.c_uint32(), C.c_uint32()
+        nodes, ents = C.c_long(), C.c_long()
+        K = C.c_double()
+        check(lib().gdt_korobov_info(self.h, C.byref(M), C.byref(a), C.byref(nodes), C.byref(ents), C.byref(K)))
+        return {"M": M.value, "a": a.value, "nodes": nodes.value, "entries": ents.value, "K": K.value}
+
+    def korobov_export(self):
+        info = self.korobov_info()
+        nn = info["nodes"]
+        idx6 = np.empty((nn, 6), np.int32)
+        r = np.empty(nn)
+        w = np.empty(nn)
+        node = np.empty(nn, np.uint32)
+        check(lib().gdt_korobov_export(self.h, idx6.ctypes.data, r.ctypes.data, w.ctypes.data, node.ctypes.data))
+        return idx6, r, w, node
+
+    def korobov_export_csr(self):
+        info = self.korobov_info()
+        ptr = np.empty(self.n_pnt + 1, np.int32)
+        ent = np.empty(info["entries"], np.uint32)
+        check(lib().gdt_korobov_export_csr(self.h, ptr.ctypes.data, ent.ctypes.data))
+        return ptr, ent
+
+    # -- operator ---------------------------------------------------------
+    def collide_matrix(self, f, cells, layout=LAYOUT_CELL_MAJOR):
+        nu = DeviceArray((cells, self.n_pnt))
+        gain = DeviceArray((cells, self.n_pnt))
+        check(lib().gdt_collide_matrix(self.h, f.ptr, nu.ptr, gain.ptr, int(cells), int(layout)))
+        return nu, gain
+
+    def collide_korobov(self, f, cells, layout=LAYOUT_CELL_MAJOR):
+        loss = DeviceArray((cells, self.n_pnt))
+        Q = DeviceArray((cells, self.n_pnt))
+        check(lib().gdt_collide_korobov(self.h, f.ptr, loss.ptr, Q.ptr, int(cells), int(layout)))
+        return loss, Q
+
+    def evaluate(self, method, f, cells, flags=FLAG_CORRECT, layout=LAYOUT_CELL_MAJOR):
+        nu = DeviceArray((cells, self.n_pnt))
+        gain = DeviceArray((cells, self.n_pnt))
+        Q = DeviceArray((cells, self.n_pnt))
+        check(lib().gdt_evaluate(self.h, int(method), f.ptr, nu.ptr, gain.ptr, Q.ptr, int(cells), int(layout),
+                                 int(flags)))
+        return nu, gain, Q
+
+    def correct(self, method, f, nu, gain, Q, cells, X=None):
+        check(lib().gdt_correct(self.h, int(method), f.ptr, nu.ptr, gain.ptr, Q.ptr, X.ptr if X else None,
+                                int(cells), LAYOUT_CELL_MAJOR))
+
+    def relax(self, f, nu, Q, cells, dt=0.0, courant=0.5, time=None, flags=0, layout=LAYOUT_CELL_MAJOR):
+        check(lib().gdt_relax(self.h, f.ptr, nu.ptr if nu else None, Q.ptr, int(cells), float(dt), float(courant),
+                              time.ptr if time else None, int(layout), int(flags)))
+
+    def run_steps(self, method, f, cells, dt, steps, flags=FLAG_CORRECT):
+        check(lib().gdt_run_steps(self.h, int(method), f.ptr, int(cells), float(dt), int(steps), int(flags)))
+
+    def kernel_stats(self, name):
+        ms, cnt = C.c_double(), C.c_long()
+        check(lib().gdt_kernel_stats(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def kernel_stats_reset(self):
+        check(lib().gdt_kernel_stats_reset(self.h))
+
+    def transpose(self, src, dst, rows, cols):
+        check(lib().gdt_transpose(self.h, src.ptr, dst.ptr, int(rows), int(cols)))
+
+
+def korobov_pick(M, seed, ncand=8):
+    p2 = C.c_double()
+    a = lib().gdt_korobov_pick(int(M), int(seed), int(ncand), C.byref(p2))
+    return int(a), p2.value

@@ -1,0 +1,829 @@
+// Collision-operator kernels for gfx950 (wave64).
+//
+// Layouts: velocity arrays are [cell][node]; "box" arrays cover the whole
+// n_u x n_v x n_w box with zeros outside the sphere cut, "cmp" arrays hold the
+// kept nodes only.  All arithmetic is FP64.
+#include <cstdio>
+
+#include "gdt_internal.h"
+
+#define WAVE 64
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// layout helpers
+// ---------------------------------------------------------------------------
+
+__global__ void k_cmp_to_box(int n_box, int n_pnt, int cells, int layout, const int *__restrict__ box2c,
+                             const double *__restrict__ src, double *__restrict__ dst)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n_box * cells) return;
+    const int c = (int)(t / n_box), b = (int)(t % n_box);
+    const int v = box2c[b];
+    double x = 0.0;
+    if (v >= 0) x = layout == GDT_LAYOUT_NODE_MAJOR ? src[(long)v * cells + c] : src[(long)c * n_pnt + v];
+    dst[t] = x;
+}
+
+__global__ void k_box_to_cmp(int n_box, int n_pnt, int cells, int layout, const int *__restrict__ cmp2box,
+                             const double *__restrict__ src, double *__restrict__ dst)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n_pnt * cells) return;
+    const int c = (int)(t / n_pnt), v = (int)(t % n_pnt);
+    const double x = src[(long)c * n_box + cmp2box[v]];
+    if (layout == GDT_LAYOUT_NODE_MAJOR)
+        dst[(long)v * cells + c] = x;
+    else
+        dst[t] = x;
+}
+
+int launch_cmp_to_box(gdt_context *h, const double *f_cmp, double *f_box, int cells, int layout)
+{
+    const long tot = (long)h->n_box * cells;
+    hipLaunchKernelGGL(k_cmp_to_box, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->n_box, h->n_pnt,
+                       cells, layout, h->box2c.as<int>(), f_cmp, f_box);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+int launch_box_to_cmp(gdt_context *h, const double *f_box, double *f_cmp, int cells, int layout)
+{
+    const long tot = (long)h->n_pnt * cells;
+    hipLaunchKernelGGL(k_box_to_cmp, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->n_box, h->n_pnt,
+                       cells, layout, h->cmp2box.as<int>(), f_box, f_cmp);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// loss frequency: 3-D Toeplitz correlation with the b table
+// ---------------------------------------------------------------------------
+// nu(i,j,k) = sum_{i1,j1,k1} b(|i-i1|,|j-j1|,|k-k1|) f(i1,j1,k1)
+// (evolve_direct_colisions, collision_integral.cu:32-72).  One lane owns a
+// destination row (i, j, 0..NW-1) of one cell in registers; per partner row
+// (i1, j1) it loads the b row of (|i-i1|, |j-j1|) and the (wave-uniform) f
+// row and performs NW*NW FMAs, so the loop is FMA bound, not load bound.
+template <int NW>
+__global__ __launch_bounds__(64) void k_loss(int nu, int nv, const double *__restrict__ b,
+                                             const double *__restrict__ f, double *__restrict__ out)
+{
+    const int c = blockIdx.y;
+    const int rows = nu * nv;
+    const int row = blockIdx.x * 64 + threadIdx.x;
+    const bool act = row < rows;
+    const int i = act ? row / nv : 0, j = act ? row % nv : 0;
+    const double *fc = f + (size_t)c * rows * NW;
+    double acc[NW];
+#pragma unroll
+    for (int k = 0; k < NW; k++) acc[k] = 0.0;
+    for (int i1 = 0; i1 < nu; i1++) {
+        const int gi = i > i1 ? i - i1 : i1 - i;
+        for (int j1 = 0; j1 < nv; j1++) {
+            const int gj = j > j1 ? j - j1 : j1 - j;
+            const double *brow = b + (gi * nv + gj) * NW;
+            const double *frow = fc + (i1 * nv + j1) * NW;
+            double bb[NW], ff[NW];
+#pragma unroll
+            for (int k = 0; k < NW; k++) {
+                bb[k] = brow[k];
+                ff[k] = frow[k];
+            }
+#pragma unroll
+            for (int k = 0; k < NW; k++)
+#pragma unroll
+                for (int k1 = 0; k1 < NW; k1++) acc[k] += bb[k > k1 ? k - k1 : k1 - k] * ff[k1];
+        }
+    }
+    if (act) {
+        double *o = out + (size_t)c * rows * NW + (size_t)row * NW;
+#pragma unroll
+        for (int k = 0; k < NW; k++) o[k] = acc[k];
+    }
+}
+
+// Generic-width fallback (any n_w): the same sum, one destination per lane.
+__global__ __launch_bounds__(256) void k_loss_generic(int nu, int nv, int nw, const double *__restrict__ b,
+                                                      const double *__restrict__ f, double *__restrict__ out)
+{
+    const int c = blockIdx.y;
+    const int nb = nu * nv * nw;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nb) return;
+    const int i = v / (nv * nw), j = (v / nw) % nv, k = v % nw;
+    const double *fc = f + (size_t)c * nb;
+    double s = 0.0;
+    for (int i1 = 0; i1 < nu; i1++)
+        for (int j1 = 0; j1 < nv; j1++) {
+            const double *brow = b + ((i > i1 ? i - i1 : i1 - i) * nv + (j > j1 ? j - j1 : j1 - j)) * nw;
+            const double *frow = fc + (i1 * nv + j1) * nw;
+            for (int k1 = 0; k1 < nw; k1++) s += brow[k > k1 ? k - k1 : k1 - k] * frow[k1];
+        }
+    out[(size_t)c * nb + v] = s;
+}
+
+int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells)
+{
+    const int nu = h->n[0], nv = h->n[1], nw = h->n[2];
+    const dim3 grid((nu * nv + 63) / 64, cells);
+    const double *b = h->tb.as<double>();
+    KernelTimer kt(h, "loss");
+    switch (nw) {
+    case 8: hipLaunchKernelGGL(k_loss<8>, grid, dim3(64), 0, h->stream, nu, nv, b, f_box, nu_box); break;
+    case 16: hipLaunchKernelGGL(k_loss<16>, grid, dim3(64), 0, h->stream, nu, nv, b, f_box, nu_box); break;
+    case 20: hipLaunchKernelGGL(k_loss<20>, grid, dim3(64), 0, h->stream, nu, nv, b, f_box, nu_box); break;
+    case 24: hipLaunchKernelGGL(k_loss<24>, grid, dim3(64), 0, h->stream, nu, nv, b, f_box, nu_box); break;
+    default:
+        hipLaunchKernelGGL(k_loss_generic, dim3((h->n_box + 255) / 256, cells), dim3(256), 0, h->stream, nu, nv, nw, b,
+                           f_box, nu_box);
+    }
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gain of the analytic matrix method
+// ---------------------------------------------------------------------------
+// gain(v) = sum_{v1} sum_{boxes e of g = |v - v1|} a_e t4(v, v1, e), with the
+// four sign terms t4 of evolve_inverse_colisions (collision_integral.cu:
+// 113-162) and the intended output slot.  f of one cell sits in LDS.  A wave
+// owns a 4x4x4 block of destinations; it walks the relative-velocity triples
+// g (wave-uniform), the 8 partner signs v1 = v +- g per axis, and the g's
+// nonzero boxes (scalar loads).  Sign choices with no partner inside the grid
+// for any lane of the block are skipped as a whole.
+__global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int *__restrict__ gptr,
+                                              const int *__restrict__ gbox, const double *__restrict__ gval,
+                                              const double *__restrict__ f, double *__restrict__ out)
+{
+    extern __shared__ double fs[];
+    const int nyz = nv * nw, nb = nu * nyz;
+    const int c = blockIdx.y;
+    const double *fc = f + (size_t)c * nb;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) fs[t] = fc[t];
+    __syncthreads();
+
+    const int bu = (nu + 3) / 4, bv = (nv + 3) / 4, bw = (nw + 3) / 4;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int blk = blockIdx.x * 4 + wave;
+    if (blk >= bu * bv * bw) return;  // whole wave exits; no barrier follows
+    const int bi = blk / (bv * bw), bj = (blk / bw) % bv, bk = blk % bw;
+    const int i = bi * 4 + lane / 16, j = bj * 4 + (lane / 4) % 4, k = bk * 4 + lane % 4;
+    const bool act = i < nu && j < nv && k < nw;
+    const int hv = nv / 2, hw = nw / 2;
+
+    double acc = 0.0;
+    for (int gi = 0; gi < nu; gi++)
+        for (int gj = 0; gj < nv; gj++)
+            for (int gk = 0; gk < nw; gk++) {
+                const int g = (gi * nv + gj) * nw + gk;
+                const int e0 = gptr[g], e1 = gptr[g + 1];
+                if (e0 == e1) continue;
+                const int pi = gi & 1, pj = gj & 1, pk = gk & 1;
+                for (int sgn = 0; sgn < 8; sgn++) {
+                    if (((sgn & 1) && gi == 0) || ((sgn & 2) && gj == 0) || ((sgn & 4) && gk == 0)) continue;
+                    const int i1 = (sgn & 1) ? i + gi : i - gi;
+                    const int j1 = (sgn & 2) ? j + gj : j - gj;
+                    const int k1 = (sgn & 4) ? k + gk : k - gk;
+                    const bool ok = act && i1 >= 0 && i1 < nu && j1 >= 0 && j1 < nv && k1 >= 0 && k1 < nw;
+                    if (!__any(ok)) continue;
+                    const int SI = i + i1, SJ = j + j1, SK = k + k1;
+                    for (int e = e0; e < e1; e++) {
+                        const int box = gbox[e];
+                        const double val = gval[e];
+                        const int kn1 = box % hw, jn1 = (box / hw) % hv, in1 = box / (hw * hv);
+                        const int in = (in1 << 1) | pi, jn = (jn1 << 1) | pj, kn = (kn1 << 1) | pk;
+                        const int ih = (SI + in) >> 1, im = SI - in;
+                        const int jh = (SJ + jn) >> 1, jm = SJ - jn;
+                        const int kh = (SK + kn) >> 1, km = SK - kn;
+                        const bool valid = ok && ih < nu && im >= 0 && jh < nv && jm >= 0 && kh < nw && km >= 0;
+                        if (valid) {
+                            const int il = im >> 1, jl = jm >> 1, kl = km >> 1;
+                            const int Ih = ih * nyz, Il = il * nyz, Jh = jh * nw, Jl = jl * nw;
+                            const double t4 = fs[Ih + Jh + kh] * fs[Il + Jl + kl] + fs[Ih + Jl + kl] * fs[Il + Jh + kh] +
+                                              fs[Ih + Jh + kl] * fs[Il + Jl + kh] + fs[Ih + Jl + kh] * fs[Il + Jh + kl];
+                            acc += val * t4;
+                        }
+                    }
+                }
+            }
+    if (act) out[(size_t)c * nb + (i * nv + j) * nw + k] = acc;
+}
+
+int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells)
+{
+    const int nu = h->n[0], nv = h->n[1], nw = h->n[2];
+    const int blocks = ((nu + 3) / 4) * ((nv + 3) / 4) * ((nw + 3) / 4);
+    const size_t lds = sizeof(double) * h->n_box;
+    GDT_CHECK(lds <= 160 * 1024, GDT_E_ARG, "gain: grid of %d box nodes exceeds the LDS budget", h->n_box);
+    KernelTimer kt(h, "gain");
+    hipLaunchKernelGGL(k_gain, dim3((blocks + 3) / 4, cells), dim3(256), lds, h->stream, nu, nv, nw,
+                       h->gptr.as<int>(), h->gbox.as<int>(), h->gval.as<double>(), f_box, gain_box);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Korobov projection method
+// ---------------------------------------------------------------------------
+// Node values: per stencil node t and cell c,
+//   P = f_a f_b,  R = f_l f_m (f_ls f_ms / f_l f_m)^r,  D = w (P - R)
+//   vals = (D, (1-r) D, r D, w P)
+// (linear interpolation of the two pair products when one is not positive).
+__global__ __launch_bounds__(256) void k_kor_values(long nn, int npts, const int32_t *__restrict__ idx6,
+                                                    const double *__restrict__ rr, const double *__restrict__ ww,
+                                                    const double *__restrict__ f, double *__restrict__ vals)
+{
+#pragma clang fp contract(off)
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn) return;
+    const int c = blockIdx.y;
+    const double *fc = f + (size_t)c * npts;
+    const int2 *ip = reinterpret_cast<const int2 *>(idx6 + 6 * t);
+    const int2 i01 = ip[0], i23 = ip[1], i45 = ip[2];
+    const double r = rr[t], w = ww[t];
+    const double P = fc[i01.x] * fc[i01.y];
+    const double Pl = fc[i23.x] * fc[i23.y];
+    double R;
+    if (r == 0.0) {
+        R = Pl;
+    } else {
+        const double Ps = fc[i45.x] * fc[i45.y];
+        if (Pl > 0.0 && Ps > 0.0)
+            R = Pl * exp(r * log(Ps / Pl));
+        else
+            R = (1.0 - r) * Pl + r * Ps;
+    }
+    const double D = w * (P - R);
+    double4 o;
+    o.x = D;
+    o.y = (1.0 - r) * D;
+    o.z = r * D;
+    o.w = w * P;
+    reinterpret_cast<double4 *>(vals)[(size_t)c * nn + t] = o;
+}
+
+int launch_kor_values(gdt_context *h, const double *f, double *vals, int cells)
+{
+    KernelTimer kt(h, "kor_values");
+    hipLaunchKernelGGL(k_kor_values, dim3((unsigned)((h->nn + 255) / 256), cells), dim3(256), 0, h->stream, h->nn,
+                       h->n_pnt, h->kidx.as<int32_t>(), h->kr.as<double>(), h->kw.as<double>(), f, vals);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// Destination-sorted segmented reduction: one wave per destination node;
+// the CSR entry (node << 2 | role) selects -D and +wP (role 0: alpha/beta),
+// (1-r) D (role 1: lambda/mu) or r D (role 2: lambda+s/mu-s).  Lanes stride
+// the segment in order and a fixed butterfly finishes, so the sum order is
+// the same on every run.
+__global__ __launch_bounds__(256) void k_kor_reduce(int npts, long nn, const int32_t *__restrict__ ptr,
+                                                    const uint32_t *__restrict__ ent,
+                                                    const double *__restrict__ vals, double *__restrict__ loss,
+                                                    double *__restrict__ Q)
+{
+#pragma clang fp contract(off)
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int gm = blockIdx.x * (blockDim.x / WAVE) + wave;
+    if (gm >= npts) return;
+    const int c = blockIdx.y;
+    const double *vc = vals + (size_t)c * nn * 4;
+    double sq = 0.0, sl = 0.0;
+    const int e1 = ptr[gm + 1];
+    for (int e = ptr[gm] + lane; e < e1; e += WAVE) {
+        const uint32_t x = ent[e];
+        const size_t t = x >> 2;
+        const uint32_t role = x & 3u;
+        if (role == 0) {
+            const double2 a = *reinterpret_cast<const double2 *>(vc + 4 * t);
+            sq -= a.x;
+            sl += vc[4 * t + 3];
+        } else {
+            sq += vc[4 * t + role];
+        }
+    }
+    sq = wave_sum(sq);
+    sl = wave_sum(sl);
+    if (lane == 0) {
+        Q[(size_t)c * npts + gm] = sq;
+        if (loss) loss[(size_t)c * npts + gm] = sl;
+    }
+}
+
+int launch_kor_reduce(gdt_context *h, const double *vals, double *loss, double *Q, int cells)
+{
+    KernelTimer kt(h, "kor_reduce");
+    hipLaunchKernelGGL(k_kor_reduce, dim3((h->n_pnt + 3) / 4, cells), dim3(256), 0, h->stream, h->n_pnt, h->nn,
+                       h->kptr.as<int32_t>(), h->kent.as<uint32_t>(), vals, loss, Q);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// conservation correction (+ optional explicit step), one block per cell
+// ---------------------------------------------------------------------------
+// With psi = (1, xi_x, xi_y, xi_z, |xi|^2) and L = f nu (MATRIX) or the loss
+// term itself (KOROBOV):
+//   B_m = sum (Q0) psi_m  with Q0 = gain - L (MATRIX) or the raw Q (KOROBOV)
+//   A_mn = sum L psi_m psi_n,  A X = B (partial pivoting),
+//   MATRIX:  nu_c = nu (1 + X.psi),  Q = gain - f nu_c
+//   KOROBOV: Q = Q0 - L X.psi,       nu_c = (L/f)(1 + X.psi)
+// (before_correction, linear_solver, correct_frequency,
+// collision_integral.cu:217-526, solved for every cell).  With dt != 0 the
+// block also applies f <- f + dt Q (evolve_f, :809-833).
+__device__ bool solve5(double M[5][6], double X[5])
+{
+    for (int col = 0; col < 5; col++) {
+        int piv = col;
+        for (int r = col + 1; r < 5; r++)
+            if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
+        if (M[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int cc = 0; cc < 6; cc++) {
+                const double t = M[col][cc];
+                M[col][cc] = M[piv][cc];
+                M[piv][cc] = t;
+            }
+        for (int r = col + 1; r < 5; r++) {
+            const double m = M[r][col] / M[col][col];
+            for (int cc = col; cc < 6; cc++) M[r][cc] -= m * M[col][cc];
+        }
+    }
+    for (int r = 4; r >= 0; r--) {
+        double s = M[r][5];
+        for (int cc = r + 1; cc < 5; cc++) s -= M[r][cc] * X[cc];
+        X[r] = s / M[r][r];
+    }
+    return true;
+}
+
+// MODE 0: matrix, corrected; 1: Korobov, corrected; 2: matrix, X = 0;
+// 3: Korobov, X = 0.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_correct(int npts, const double *__restrict__ xi, const double *f,
+                                                 const double *__restrict__ nu, const double *__restrict__ gain,
+                                                 double *__restrict__ nu_out, double *__restrict__ Q,
+                                                 double *__restrict__ Xout, double dt, double *f_upd)
+{
+    constexpr bool KOR = (MODE == 1 || MODE == 3);
+    constexpr bool SOLVE = (MODE == 0 || MODE == 1);
+    __shared__ double part[4][20];
+    __shared__ double Xs[5];
+    const int c = blockIdx.x;
+    const size_t o = (size_t)c * npts;
+    if (SOLVE) {
+        double m[20];
+#pragma unroll
+        for (int q = 0; q < 20; q++) m[q] = 0.0;
+        for (int v = threadIdx.x; v < npts; v += blockDim.x) {
+            const double x = xi[3 * v], y = xi[3 * v + 1], z = xi[3 * v + 2];
+            const double e = x * x + y * y + z * z;
+            const double psi[5] = {1.0, x, y, z, e};
+            double L, R;
+            if (KOR) {
+                L = nu[o + v];
+                R = gain[o + v];
+            } else {
+                L = nu[o + v] * f[o + v];
+                R = gain[o + v] - L;
+            }
+#pragma unroll
+            for (int a = 0; a < 5; a++) m[a] += R * psi[a];
+            int q = 5;
+#pragma unroll
+            for (int a = 0; a < 5; a++)
+#pragma unroll
+                for (int b = a; b < 5; b++) m[q++] += L * psi[a] * psi[b];
+        }
+        const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+#pragma unroll
+        for (int q = 0; q < 20; q++) {
+            const double s = wave_sum(m[q]);
+            if (lane == 0) part[wave][q] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int nwv = blockDim.x / WAVE;
+            double s[20];
+            for (int q = 0; q < 20; q++) {
+                s[q] = 0.0;
+                for (int w = 0; w < nwv; w++) s[q] += part[w][q];
+            }
+            double M[5][6];
+            int q = 5;
+            for (int a = 0; a < 5; a++)
+                for (int b = a; b < 5; b++) {
+                    M[a][b] = s[q];
+                    M[b][a] = s[q];
+                    q++;
+                }
+            for (int a = 0; a < 5; a++) M[a][5] = s[a];
+            double X[5] = {0, 0, 0, 0, 0};
+            if (!solve5(M, X))
+                for (int a = 0; a < 5; a++) X[a] = 0.0;
+            for (int a = 0; a < 5; a++) Xs[a] = X[a];
+        }
+        __syncthreads();
+    } else if (threadIdx.x < 5) {
+        Xs[threadIdx.x] = 0.0;
+    }
+    __syncthreads();
+    if (Xout && threadIdx.x < 5) Xout[5 * c + threadIdx.x] = Xs[threadIdx.x];
+    const double X0 = Xs[0], X1 = Xs[1], X2 = Xs[2], X3 = Xs[3], X4 = Xs[4];
+    for (int v = threadIdx.x; v < npts; v += blockDim.x) {
+        const double x = xi[3 * v], y = xi[3 * v + 1], z = xi[3 * v + 2];
+        const double e = x * x + y * y + z * z;
+        const double fv = f[o + v];
+        double q;
+        if (KOR) {
+            const double s = X0 + X1 * x + X2 * y + X3 * z + X4 * e;
+            const double L = nu[o + v];
+            q = gain[o + v] - L * s;
+            if (nu_out) nu_out[o + v] = fv > 0.0 ? (L / fv) * (1.0 + s) : 0.0;
+        } else {
+            const double s = 1.0 + X0 + X1 * x + X2 * y + X3 * z + X4 * e;
+            const double nc = nu[o + v] * s;
+            q = gain[o + v] - fv * nc;
+            if (nu_out) nu_out[o + v] = nc;
+        }
+        Q[o + v] = q;
+        if (dt != 0.0) f_upd[o + v] = fv + dt * q;
+    }
+}
+
+// gain = Q + f nu (the gain of the Korobov form after correction).
+__global__ void k_compose_gain(long count, const double *__restrict__ f, const double *__restrict__ nu,
+                               const double *__restrict__ Q, double *__restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = Q[t] + f[t] * nu[t];
+}
+
+// method: GDT_METHOD_MATRIX / GDT_METHOD_KOROBOV = corrected; -1 / -2 = the
+// same without correction (X = 0); -3 = gain composition out = Q + f nu with
+// (nu, gain) = (nu, Q) and out = Q argument.
+int launch_correct(gdt_context *h, int method, const double *f, const double *nu, const double *gain, double *nu_out,
+                   double *Q, double *X, int cells, double dt, double *f_upd)
+{
+    if (method == -3) {
+        const long count = (long)cells * h->n_pnt;
+        hipLaunchKernelGGL(k_compose_gain, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->stream, count, f, nu,
+                           gain, Q);
+        GDT_HIP(hipGetLastError());
+        return GDT_OK;
+    }
+    KernelTimer kt(h, "correct");
+    const dim3 g(cells), b(256);
+    const double *xi = h->xi.as<double>();
+    const int np = h->n_pnt;
+    switch (method) {
+    case GDT_METHOD_MATRIX:
+        hipLaunchKernelGGL(k_correct<0>, g, b, 0, h->stream, np, xi, f, nu, gain, nu_out, Q, X, dt, f_upd);
+        break;
+    case GDT_METHOD_KOROBOV:
+        hipLaunchKernelGGL(k_correct<1>, g, b, 0, h->stream, np, xi, f, nu, gain, nu_out, Q, X, dt, f_upd);
+        break;
+    case -1:
+        hipLaunchKernelGGL(k_correct<2>, g, b, 0, h->stream, np, xi, f, nu, gain, nu_out, Q, X, dt, f_upd);
+        break;
+    case -2:
+        hipLaunchKernelGGL(k_correct<3>, g, b, 0, h->stream, np, xi, f, nu, gain, nu_out, Q, X, dt, f_upd);
+        break;
+    default: gdt_set_error(GDT_E_ARG, "launch_correct: method %d", method); return GDT_E_ARG;
+    }
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// explicit step helpers (relax_f, collision_integral.cu:776-880)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_courant_dt(const double *__restrict__ nu, long count, double courant,
+                                                     double *__restrict__ dt)
+{
+    __shared__ double red[1024 / WAVE];
+    double m = 1.0e10;
+    for (long t = threadIdx.x; t < count; t += blockDim.x) {
+        const double v = nu[t];
+        if (v > 0.0) {
+            const double d = courant / v;
+            m = d < m ? d : m;
+        }
+    }
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, WAVE);
+        m = o < m ? o : m;
+    }
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x / WAVE); w++) r = red[w] < r ? red[w] : r;
+        dt[0] = r;
+    }
+}
+
+int launch_courant_dt(gdt_context *h, const double *nu, long count, double courant, double *dt_out)
+{
+    hipLaunchKernelGGL(k_courant_dt, dim3(1), dim3(1024), 0, h->stream, nu, count, courant, dt_out);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+__global__ void k_euler(double *__restrict__ f, const double *__restrict__ Q, long count, double dt,
+                        const double *__restrict__ dt_dev, double *__restrict__ time)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double d = dt_dev ? dt_dev[0] : dt;
+    if (t < count) f[t] = f[t] + d * Q[t];
+    if (t == 0 && time) time[0] += d;
+}
+
+int launch_euler(gdt_context *h, double *f, const double *Q, long count, double dt, const double *dt_dev, double *time)
+{
+    hipLaunchKernelGGL(k_euler, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->stream, f, Q, count, dt, dt_dev,
+                       time);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// [rows][cols] -> [cols][rows] through a padded LDS tile (transpose_float,
+// collision_integral.cu:528-547).
+__global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ src, double *__restrict__ dst, int rows,
+                                                   int cols)
+{
+    __shared__ double tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    for (int r = ty; r < 32; r += 8) {
+        const int row = by + r, col = bx + tx;
+        if (row < rows && col < cols) tile[r][tx] = src[(size_t)row * cols + col];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int col = bx + r, row = by + tx;
+        if (row < rows && col < cols) dst[(size_t)col * rows + row] = tile[tx][r];
+    }
+}
+
+int launch_transpose(gdt_context *h, const double *src, double *dst, int rows, int cols)
+{
+    hipLaunchKernelGGL(k_transpose, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0, h->stream, src, dst, rows,
+                       cols);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// The 14 moment-basis planes of fill_correction_array (velocity.cu:17-50)
+// over the box: xi_x, xi_y, xi_z, xi^2, xi_x xi_x, xi_x xi_y, xi_x xi_z,
+// xi_x xi^2, xi_y xi_y, xi_y xi_z, xi_y xi^2, xi_z xi_z, xi_z xi^2, xi^4.
+__global__ void k_basis(int nu, int nv, int nw, double u0, double v0, double w0, double du, double dv, double dw,
+                        double *__restrict__ dst)
+{
+    const int nb = nu * nv * nw;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const int i = t / (nv * nw), j = (t / nw) % nv, k = t % nw;
+    const double x = u0 + du * (2.0 * i + 1.0) / 2.0;
+    const double y = v0 + dv * (2.0 * j + 1.0) / 2.0;
+    const double z = w0 + dw * (2.0 * k + 1.0) / 2.0;
+    const double e = x * x + y * y + z * z;
+    const double p[14] = {x, y, z, e, x * x, x * y, x * z, x * e, y * y, y * z, y * e, z * z, z * e, e * e};
+#pragma unroll
+    for (int q = 0; q < 14; q++) dst[(size_t)q * nb + t] = p[q];
+}
+
+int launch_basis(gdt_context *h, double *dst)
+{
+    hipLaunchKernelGGL(k_basis, dim3((h->n_box + 255) / 256), dim3(256), 0, h->stream, h->n[0], h->n[1], h->n[2],
+                       h->vmin[0], h->vmin[1], h->vmin[2], h->d[0], h->d[1], h->d[2], dst);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+__global__ void k_fill(double *dst, long count, double value)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) dst[t] = value;
+}
+
+int launch_fill(gdt_context *h, double *dst, long count, double value)
+{
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->stream, dst, count, value);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fused projection-method collide (f in LDS, entry records streamed)
+// ---------------------------------------------------------------------------
+// The stencil is stored a second time in CSR order, one 20-byte record per
+// (destination, node, role) entry: the six node indices as uint16 pairs, the
+// integers E0 - E_lm and E_s - E_lm that fix r = num/den, and D2 | role << 16
+// that fixes w = K sqrt(D2).  A block stages f of its cell in LDS; each wave
+// owns destinations and streams their records (coalesced), recomputing the
+// node's Delta per entry, so nothing is gathered from HBM at random.  The
+// per-entry arithmetic is the one of k_kor_values / k_kor_reduce operation for
+// operation (contraction off), so both paths give the same bits.  Each block
+// also writes the 20 moment sums of its destinations for the correction.
+#define FUSED_THREADS 1024
+
+__global__ __launch_bounds__(FUSED_THREADS) void k_kor_fused(int npts, int dpb, const int32_t *__restrict__ ptr,
+                                                              const uint4 *__restrict__ e4,
+                                                              const uint32_t *__restrict__ e1, double K,
+                                                              const double *__restrict__ f,
+                                                              const double *__restrict__ xi,
+                                                              double *__restrict__ loss, double *__restrict__ Q,
+                                                              double *__restrict__ mpart)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double fs[];
+    __shared__ double wpart[FUSED_THREADS / WAVE][20];
+    const int c = blockIdx.y;
+    const double *fc = f + (size_t)c * npts;
+    for (int t = threadIdx.x; t < npts; t += blockDim.x) fs[t] = fc[t];
+    __syncthreads();
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nwv = blockDim.x / WAVE;
+    const int g0 = blockIdx.x * dpb;
+    const int g1 = min(npts, g0 + dpb);
+    double m[20];
+#pragma unroll
+    for (int q = 0; q < 20; q++) m[q] = 0.0;
+    for (int gm = g0 + wave; gm < g1; gm += nwv) {
+        double sq = 0.0, sl = 0.0;
+        const int eend = ptr[gm + 1];
+        for (int e = ptr[gm] + lane; e < eend; e += WAVE) {
+            const uint4 x = e4[e];
+            const uint32_t y = e1[e];
+            const int num = (int)(short)(x.w & 0xFFFFu), den = (int)(short)(x.w >> 16);
+            const double P = fs[x.x & 0xFFFFu] * fs[x.x >> 16];
+            const double Pl = fs[x.y & 0xFFFFu] * fs[x.y >> 16];
+            double r = 0.0, R;
+            if (num == 0) {
+                R = Pl;
+            } else {
+                r = (double)num / (double)den;
+                const double Ps = fs[x.z & 0xFFFFu] * fs[x.z >> 16];
+                if (Pl > 0.0 && Ps > 0.0)
+                    R = Pl * exp(r * log(Ps / Pl));
+                else
+                    R = (1.0 - r) * Pl + r * Ps;
+            }
+            const double w = K * sqrt((double)(y & 0xFFFFu));
+            const double D = w * (P - R);
+            const uint32_t role = y >> 16;
+            if (role == 0) {
+                sq -= D;
+                const double wp = w * P;
+                sl += wp;
+            } else if (role == 1) {
+                const double t1 = (1.0 - r) * D;
+                sq += t1;
+            } else {
+                const double t2 = r * D;
+                sq += t2;
+            }
+        }
+        sq = wave_sum(sq);
+        sl = wave_sum(sl);
+        if (lane == 0) {
+            Q[(size_t)c * npts + gm] = sq;
+            if (loss) loss[(size_t)c * npts + gm] = sl;
+            const double x0 = xi[3 * gm], y0 = xi[3 * gm + 1], z0 = xi[3 * gm + 2];
+            const double psi[5] = {1.0, x0, y0, z0, x0 * x0 + y0 * y0 + z0 * z0};
+#pragma unroll
+            for (int a = 0; a < 5; a++) m[a] += sq * psi[a];
+            int q = 5;
+#pragma unroll
+            for (int a = 0; a < 5; a++)
+#pragma unroll
+                for (int b = a; b < 5; b++) m[q++] += sl * psi[a] * psi[b];
+        }
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 20; q++) wpart[wave][q] = m[q];
+    __syncthreads();
+    if (threadIdx.x < 20) {
+        double s = 0.0;
+        for (int w = 0; w < nwv; w++) s += wpart[w][threadIdx.x];
+        mpart[((size_t)c * gridDim.x + blockIdx.x) * 20 + threadIdx.x] = s;
+    }
+}
+
+int kor_fused_blocks(gdt_context *h, int cells)
+{
+    (void)h;
+    const int target = 256;  // one block per CU at one block of LDS per CU
+    int b = (target + cells - 1) / cells;
+    return b < 1 ? 1 : b;
+}
+
+// Called once when the entry records are built (not during graph capture).
+int kor_fused_setup(gdt_context *h)
+{
+    const int lds = (int)(sizeof(double) * h->n_pnt);
+    GDT_HIP(hipFuncSetAttribute((const void *)k_kor_fused, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    return GDT_OK;
+}
+
+int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells)
+{
+    const int B = kor_fused_blocks(h, cells);
+    const int dpb = (h->n_pnt + B - 1) / B;
+    const size_t lds = sizeof(double) * h->n_pnt;
+    KernelTimer kt(h, "kor_fused");
+    hipLaunchKernelGGL(k_kor_fused, dim3(B, cells), dim3(FUSED_THREADS), lds, h->stream, h->n_pnt, dpb,
+                       h->kptr.as<int32_t>(), h->ke4.as<uint4>(), h->ke1.as<uint32_t>(), h->kK, f,
+                       h->xi.as<double>(), loss, Q, mpart);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// Correction from the fused kernel's moment partials, then (dt != 0) the
+// explicit step.  Every block reduces the partials of its cell in the same
+// order and solves the same 5x5 system, then handles 256 nodes.
+template <bool CORR>
+__global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, const double *__restrict__ mpart,
+                                                   const double *__restrict__ xi, const double *f,
+                                                   const double *__restrict__ loss, const double *__restrict__ Qin,
+                                                   double *__restrict__ Qout, double *__restrict__ nu_out,
+                                                   double *__restrict__ Xout, double dt, double *f_upd)
+{
+    __shared__ double part[4][20];
+    __shared__ double Xs[5];
+    const int c = blockIdx.y;
+    if (CORR) {
+        double m[20];
+#pragma unroll
+        for (int q = 0; q < 20; q++) m[q] = 0.0;
+        for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+            const double *mp = mpart + ((size_t)c * nparts + p) * 20;
+#pragma unroll
+            for (int q = 0; q < 20; q++) m[q] += mp[q];
+        }
+        const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+#pragma unroll
+        for (int q = 0; q < 20; q++) {
+            const double s = wave_sum(m[q]);
+            if (lane == 0) part[wave][q] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s[20];
+            for (int q = 0; q < 20; q++) s[q] = part[0][q] + part[1][q] + part[2][q] + part[3][q];
+            double M[5][6];
+            int q = 5;
+            for (int a = 0; a < 5; a++)
+                for (int b = a; b < 5; b++) {
+                    M[a][b] = s[q];
+                    M[b][a] = s[q];
+                    q++;
+                }
+            for (int a = 0; a < 5; a++) M[a][5] = s[a];
+            double X[5] = {0, 0, 0, 0, 0};
+            if (!solve5(M, X))
+                for (int a = 0; a < 5; a++) X[a] = 0.0;
+            for (int a = 0; a < 5; a++) Xs[a] = X[a];
+        }
+    } else if (threadIdx.x < 5) {
+        Xs[threadIdx.x] = 0.0;
+    }
+    __syncthreads();
+    if (Xout && blockIdx.x == 0 && threadIdx.x < 5) Xout[5 * c + threadIdx.x] = Xs[threadIdx.x];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= npts) return;
+    const size_t o = (size_t)c * npts + v;
+    const double x = xi[3 * v], y = xi[3 * v + 1], z = xi[3 * v + 2];
+    const double e = x * x + y * y + z * z;
+    const double s = Xs[0] + Xs[1] * x + Xs[2] * y + Xs[3] * z + Xs[4] * e;
+    const double L = loss[o];
+    const double fv = f[o];
+    const double q = Qin[o] - L * s;
+    Qout[o] = q;
+    if (nu_out) nu_out[o] = fv > 0.0 ? (L / fv) * (1.0 + s) : 0.0;
+    if (dt != 0.0) f_upd[o] = fv + dt * q;
+}
+
+int launch_kor_apply(gdt_context *h, bool correct, const double *mpart, const double *f, const double *loss,
+                     const double *Qin, double *Qout, double *nu_out, double *X, int cells, double dt, double *f_upd)
+{
+    const int nparts = kor_fused_blocks(h, cells);
+    const dim3 grid((h->n_pnt + 255) / 256, cells);
+    KernelTimer kt(h, "kor_apply");
+    if (correct)
+        hipLaunchKernelGGL(k_kor_apply<true>, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, mpart,
+                           h->xi.as<double>(), f, loss, Qin, Qout, nu_out, X, dt, f_upd);
+    else
+        hipLaunchKernelGGL(k_kor_apply<false>, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, mpart,
+                           h->xi.as<double>(), f, loss, Qin, Qout, nu_out, X, dt, f_upd);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}

@@ -1,0 +1,826 @@
+// C ABI of libgdt.so (declarations and contracts: include/gdt.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gdt_internal.h"
+
+static thread_local std::string g_last_error;
+
+void gdt_set_error(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = std::string(buf) + " [status " + std::to_string(code) + "]";
+}
+
+int DevBuf::ensure(size_t need)
+{
+    if (need <= bytes && p) return 0;
+    release();
+    if (need == 0) need = 8;
+    if (hipMalloc(&p, need) != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        return -1;
+    }
+    bytes = need;
+    return 0;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+KernelTimer::KernelTimer(gdt_context *h_, const char *name_) : h(h_), name(name_)
+{
+    if (h->timing) (void)hipEventRecord(h->ev0, h->stream);
+}
+
+KernelTimer::~KernelTimer()
+{
+    if (!h->timing) return;
+    (void)hipEventRecord(h->ev1, h->stream);
+    (void)hipEventSynchronize(h->ev1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    KernelStat &s = h->stats[name];
+    s.ms += ms;
+    s.count += 1;
+}
+
+#define H_CHECK(h)                                                   \
+    do {                                                             \
+        if (!(h)) {                                                  \
+            gdt_set_error(GDT_E_ARG, "null gdt handle");             \
+            return GDT_E_ARG;                                        \
+        }                                                            \
+        GDT_HIP(hipSetDevice((h)->device));                          \
+    } while (0)
+
+extern "C" {
+
+const char *gdt_last_error(void) { return g_last_error.c_str(); }
+
+int gdt_version(void) { return 10000; }
+
+int gdt_device_count(int *count)
+{
+    GDT_CHECK(count, GDT_E_ARG, "count is null");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c = 0;
+    }
+    *count = c;
+    return GDT_OK;
+}
+
+int gdt_malloc(void **dptr, size_t bytes)
+{
+    GDT_CHECK(dptr, GDT_E_ARG, "dptr is null");
+    GDT_HIP(hipMalloc(dptr, bytes ? bytes : 8));
+    return GDT_OK;
+}
+
+int gdt_free(void *dptr)
+{
+    if (dptr) GDT_HIP(hipFree(dptr));
+    return GDT_OK;
+}
+
+int gdt_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    GDT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GDT_OK;
+}
+
+int gdt_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    GDT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GDT_OK;
+}
+
+int gdt_memcpy_dtod(void *dst, const void *src, size_t bytes)
+{
+    GDT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return GDT_OK;
+}
+
+int gdt_memset(void *dptr, int value, size_t bytes)
+{
+    GDT_HIP(hipMemset(dptr, value, bytes));
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+int gdt_create(gdt_handle *out, const gdt_grid_desc *gd, int device)
+{
+    GDT_CHECK(out && gd, GDT_E_ARG, "gdt_create: null argument");
+    *out = nullptr;
+    for (int a = 0; a < 3; a++) {
+        GDT_CHECK(gd->n[a] > 0 && gd->n[a] <= 128, GDT_E_ARG, "gdt_create: n[%d] = %d out of range 1..128", a,
+                  gd->n[a]);
+        GDT_CHECK(gd->vmin[a] < gd->vmax[a], GDT_E_ARG, "gdt_create: vmin[%d] >= vmax[%d]", a, a);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        gdt_set_error(GDT_E_NODEVICE, "gdt_create: no HIP device visible");
+        return GDT_E_NODEVICE;
+    }
+    GDT_CHECK(device >= 0 && device < ndev, GDT_E_ARG, "gdt_create: device %d of %d", device, ndev);
+    GDT_HIP(hipSetDevice(device));
+
+    gdt_context *h = new gdt_context();
+    h->device = device;
+    for (int a = 0; a < 3; a++) {
+        h->n[a] = gd->n[a];
+        h->vmin[a] = gd->vmin[a];
+        h->vmax[a] = gd->vmax[a];
+        h->d[a] = (gd->vmax[a] - gd->vmin[a]) / gd->n[a];
+    }
+    h->R = gd->R;
+    h->d3v = h->d[0] * h->d[1] * h->d[2];
+    const int n0 = h->n[0], n1 = h->n[1], n2 = h->n[2];
+    h->n_box = n0 * n1 * n2;
+    // sphere cut (velocity_grid.cu:6-14: centre from the box middle) and the
+    // compact node order (velocity_grid.cu:114-126)
+    h->h_box2c.assign(h->n_box, -1);
+    int cnt = 0, idx = 0;
+    for (int i = 0; i < n0; i++)
+        for (int j = 0; j < n1; j++)
+            for (int k = 0; k < n2; k++, idx++) {
+                const double x = (gd->vmax[0] - gd->vmin[0]) * (2 * i + 1 - n0) / 2.0 / n0;
+                const double y = (gd->vmax[1] - gd->vmin[1]) * (2 * j + 1 - n1) / 2.0 / n1;
+                const double z = (gd->vmax[2] - gd->vmin[2]) * (2 * k + 1 - n2) / 2.0 / n2;
+                const bool keep = (gd->R <= 0.0) || (x * x + y * y + z * z < gd->R * gd->R);
+                if (keep) {
+                    h->h_box2c[idx] = cnt++;
+                    h->h_cidx.push_back(i);
+                    h->h_cidx.push_back(j);
+                    h->h_cidx.push_back(k);
+                }
+            }
+    h->n_pnt = cnt;
+    if (cnt == 0) {
+        delete h;
+        gdt_set_error(GDT_E_ARG, "gdt_create: the sphere cut keeps no node");
+        return GDT_E_ARG;
+    }
+    std::vector<int> cmp2box(cnt);
+    h->h_xi.resize(3 * (size_t)cnt);
+    for (int v = 0; v < cnt; v++) {
+        const int i = h->h_cidx[3 * v], j = h->h_cidx[3 * v + 1], k = h->h_cidx[3 * v + 2];
+        cmp2box[v] = (i * n1 + j) * n2 + k;
+        h->h_xi[3 * v] = gd->vmin[0] + h->d[0] * (2.0 * i + 1.0) / 2.0;
+        h->h_xi[3 * v + 1] = gd->vmin[1] + h->d[1] * (2.0 * j + 1.0) / 2.0;
+        h->h_xi[3 * v + 2] = gd->vmin[2] + h->d[2] * (2.0 * k + 1.0) / 2.0;
+    }
+    int rc = GDT_OK;
+    if (h->box2c.ensure(sizeof(int) * h->n_box) || h->cidx.ensure(sizeof(int) * 3 * cnt) ||
+        h->xi.ensure(sizeof(double) * 3 * cnt) || h->cmp2box.ensure(sizeof(int) * cnt)) {
+        gdt_set_error(GDT_E_NOMEM, "gdt_create: device allocation failed");
+        rc = GDT_E_NOMEM;
+    }
+    if (rc == GDT_OK && (hipStreamCreate(&h->stream) != hipSuccess ||
+                         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)) {
+        gdt_set_error(GDT_E_HIP, "gdt_create: stream/event creation failed");
+        rc = GDT_E_HIP;
+    }
+    if (rc == GDT_OK &&
+        (hipMemcpy(h->box2c.p, h->h_box2c.data(), sizeof(int) * h->n_box, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(h->cidx.p, h->h_cidx.data(), sizeof(int) * 3 * cnt, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(h->xi.p, h->h_xi.data(), sizeof(double) * 3 * cnt, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(h->cmp2box.p, cmp2box.data(), sizeof(int) * cnt, hipMemcpyHostToDevice) != hipSuccess)) {
+        gdt_set_error(GDT_E_HIP, "gdt_create: upload failed");
+        rc = GDT_E_HIP;
+    }
+    if (rc != GDT_OK) {
+        gdt_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return GDT_OK;
+}
+
+int gdt_destroy(gdt_handle h)
+{
+    if (!h) return GDT_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    DevBuf *bufs[] = {&h->box2c, &h->cidx, &h->xi, &h->cmp2box, &h->tb, &h->ta, &h->gptr, &h->gbox, &h->gval,
+                      &h->kidx, &h->kr, &h->kw, &h->knode, &h->kptr, &h->kent, &h->s_fbox, &h->s_out1, &h->s_out2,
+                      &h->s_vals, &h->s_loss, &h->s_Q, &h->s_nu, &h->s_tmp, &h->s_dt};
+    for (DevBuf *b : bufs) b->release();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return GDT_OK;
+}
+
+int gdt_synchronize(gdt_handle h)
+{
+    H_CHECK(h);
+    GDT_HIP(hipStreamSynchronize(h->stream));
+    return GDT_OK;
+}
+
+int gdt_grid_info(gdt_handle h, int *n_pnt, int *n_box, double d[3], double *d3v)
+{
+    H_CHECK(h);
+    if (n_pnt) *n_pnt = h->n_pnt;
+    if (n_box) *n_box = h->n_box;
+    if (d)
+        for (int a = 0; a < 3; a++) d[a] = h->d[a];
+    if (d3v) *d3v = h->d3v;
+    return GDT_OK;
+}
+
+int gdt_grid_nodes(gdt_handle h, int *cidx, double *xi)
+{
+    H_CHECK(h);
+    if (cidx) std::memcpy(cidx, h->h_cidx.data(), sizeof(int) * 3 * h->n_pnt);
+    if (xi) std::memcpy(xi, h->h_xi.data(), sizeof(double) * 3 * h->n_pnt);
+    return GDT_OK;
+}
+
+int gdt_basis(gdt_handle h, double *dst)
+{
+    H_CHECK(h);
+    GDT_CHECK(dst, GDT_E_ARG, "gdt_basis: dst is null");
+    return launch_basis(h, dst);
+}
+
+// ---------------------------------------------------------------------------
+// analytic matrix tables
+// ---------------------------------------------------------------------------
+
+int gdt_build_tables(gdt_handle h, int mode)
+{
+    H_CHECK(h);
+    GDT_CHECK(mode == GDT_TABLES_FP64 || mode == GDT_TABLES_REFERENCE, GDT_E_ARG, "gdt_build_tables: mode %d", mode);
+    for (int a = 0; a < 3; a++)
+        GDT_CHECK(h->n[a] % 2 == 0, GDT_E_ARG, "gdt_build_tables: n[%d] = %d must be even", a, h->n[a]);
+    GDT_CHECK(h->n_box <= 32768, GDT_E_ARG, "gdt_build_tables: %d box nodes (max 32768)", h->n_box);
+    const int rc = launch_build_tables(h, mode);
+    if (rc != GDT_OK) return rc;
+    h->table_mode = mode;
+    h->have_tables = true;
+    return GDT_OK;
+}
+
+int gdt_tables(gdt_handle h, const double **b, const double **a, long *n_a, long *nnz)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_tables, GDT_E_STATE, "gdt_tables: tables not built");
+    if (b) *b = h->tb.as<double>();
+    if (a) *a = h->ta.as<double>();
+    if (n_a) *n_a = h->n_a;
+    if (nnz) *nnz = h->n_gent;
+    return GDT_OK;
+}
+
+// Rebuilds the nonzero-box lists from caller tables (FP64, device).
+int gdt_set_tables(gdt_handle h, const double *b, const double *a)
+{
+    H_CHECK(h);
+    GDT_CHECK(b && a, GDT_E_ARG, "gdt_set_tables: null table");
+    for (int ax = 0; ax < 3; ax++)
+        GDT_CHECK(h->n[ax] % 2 == 0, GDT_E_ARG, "gdt_set_tables: n[%d] must be even", ax);
+    const int nb = h->n_box;
+    const long n_a = (long)nb * nb / 8;
+    GDT_CHECK(h->tb.ensure(sizeof(double) * nb) == 0 && h->ta.ensure(sizeof(double) * n_a) == 0, GDT_E_NOMEM,
+              "gdt_set_tables: allocation failed");
+    GDT_HIP(hipMemcpyAsync(h->tb.p, b, sizeof(double) * nb, hipMemcpyDeviceToDevice, h->stream));
+    GDT_HIP(hipMemcpyAsync(h->ta.p, a, sizeof(double) * n_a, hipMemcpyDeviceToDevice, h->stream));
+    // derive the lists on the host (import path; not on the hot path)
+    std::vector<double> ha(n_a);
+    GDT_HIP(hipMemcpyAsync(ha.data(), a, sizeof(double) * n_a, hipMemcpyDeviceToHost, h->stream));
+    GDT_HIP(hipStreamSynchronize(h->stream));
+    const int nu = h->n[0], nv = h->n[1], nw = h->n[2], nyz = nv * nw, hu = nu / 2, hv = nv / 2, hw = nw / 2;
+    std::vector<int> gp(nb + 1), gb;
+    std::vector<double> gv;
+    for (int g = 0; g < nb; g++) {
+        gp[g] = (int)gb.size();
+        const int gk = g % nw, gj = (g / nw) % nv, gi = g / nyz;
+        for (int in1 = 0; in1 < hu; in1++)
+            for (int jn1 = 0; jn1 < hv; jn1++)
+                for (int kn1 = 0; kn1 < hw; kn1++) {
+                    const double v =
+                        ha[((long)((in1 * nu + gi) * (hv * hw) + jn1 * hw + kn1)) * nyz + gj * nw + gk];
+                    if (v != 0.0) {
+                        gb.push_back((in1 * hv + jn1) * hw + kn1);
+                        gv.push_back(v);
+                    }
+                }
+    }
+    gp[nb] = (int)gb.size();
+    const size_t ne = std::max<size_t>(gb.size(), 1);
+    GDT_CHECK(h->gptr.ensure(sizeof(int) * (nb + 1)) == 0 && h->gbox.ensure(sizeof(int) * ne) == 0 &&
+                  h->gval.ensure(sizeof(double) * ne) == 0,
+              GDT_E_NOMEM, "gdt_set_tables: list allocation failed");
+    GDT_HIP(hipMemcpy(h->gptr.p, gp.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+    if (!gb.empty()) {
+        GDT_HIP(hipMemcpy(h->gbox.p, gb.data(), sizeof(int) * gb.size(), hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->gval.p, gv.data(), sizeof(double) * gv.size(), hipMemcpyHostToDevice));
+    }
+    h->n_a = n_a;
+    h->n_gent = (long)gb.size();
+    h->table_mode = -1;
+    h->have_tables = true;
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Korobov stencil
+// ---------------------------------------------------------------------------
+
+uint32_t gdt_korobov_pick(uint32_t M, uint64_t seed, int ncand, double *p2)
+{
+    if (M < 4 || (M & (M - 1)) || ncand < 1) return 0;
+    return host_korobov_pick(M, seed, ncand, p2);
+}
+
+int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, double s0)
+{
+    H_CHECK(h);
+    GDT_CHECK(M >= 1024 && M <= (1u << 26) && (M & (M - 1)) == 0, GDT_E_ARG,
+              "gdt_korobov_build: M = %u must be a power of two in [2^10, 2^26]", M);
+    GDT_CHECK(h->n[0] == h->n[1] && h->n[1] == h->n[2], GDT_E_ARG, "gdt_korobov_build: grid must be cubic");
+    GDT_CHECK(std::fabs(h->d[0] - h->d[1]) <= 1e-14 * h->d[0] && std::fabs(h->d[0] - h->d[2]) <= 1e-14 * h->d[0],
+              GDT_E_ARG, "gdt_korobov_build: spacing must be equal on all axes");
+    GDT_CHECK(s0 > 0.0, GDT_E_ARG, "gdt_korobov_build: s0 must be positive");
+    GDT_CHECK(a == 0 || (a & 1u), GDT_E_ARG, "gdt_korobov_build: generator must be odd");
+    if (a == 0) a = host_korobov_pick(M, seed, 8, nullptr);
+    const double d = h->d[0];
+    const double K = 2.0 * 3.141592653589793 * s0 * (double)h->n_box * (double)h->n_box * d * d * d * d / (double)M;
+
+    DevBuf t_idx, t_r, t_w, t_v, t_i;
+    GDT_CHECK(t_idx.ensure(sizeof(int32_t) * 6 * (size_t)M) == 0 && t_r.ensure(sizeof(double) * M) == 0 &&
+                  t_w.ensure(sizeof(double) * M) == 0 && t_v.ensure(M) == 0 &&
+                  t_i.ensure(sizeof(int32_t) * 3 * (size_t)M) == 0,
+              GDT_E_NOMEM, "gdt_korobov_build: temporary allocation failed");
+    int rc = launch_korobov_nodes(h, M, a, K, t_idx.as<int32_t>(), t_r.as<double>(), t_w.as<double>(),
+                                  t_i.as<int32_t>(), t_v.as<uint8_t>());
+    if (rc != GDT_OK) return rc;
+    std::vector<uint8_t> hv(M);
+    std::vector<int32_t> hidx(6 * (size_t)M), hint(3 * (size_t)M);
+    std::vector<double> hr(M), hw(M);
+    GDT_HIP(hipMemcpyAsync(hv.data(), t_v.p, M, hipMemcpyDeviceToHost, h->stream));
+    GDT_HIP(hipMemcpyAsync(hidx.data(), t_idx.p, sizeof(int32_t) * 6 * (size_t)M, hipMemcpyDeviceToHost, h->stream));
+    GDT_HIP(hipMemcpyAsync(hint.data(), t_i.p, sizeof(int32_t) * 3 * (size_t)M, hipMemcpyDeviceToHost, h->stream));
+    GDT_HIP(hipMemcpyAsync(hr.data(), t_r.p, sizeof(double) * M, hipMemcpyDeviceToHost, h->stream));
+    GDT_HIP(hipMemcpyAsync(hw.data(), t_w.p, sizeof(double) * M, hipMemcpyDeviceToHost, h->stream));
+    GDT_HIP(hipStreamSynchronize(h->stream));
+    t_idx.release();
+    t_r.release();
+    t_w.release();
+    t_v.release();
+    t_i.release();
+
+    // keep the valid nodes in lattice order
+    std::vector<int32_t> cidx6, cint;
+    std::vector<double> cr, cw;
+    std::vector<uint32_t> cnode;
+    for (uint32_t t = 0; t < M; t++)
+        if (hv[t]) {
+            cidx6.insert(cidx6.end(), hidx.begin() + 6 * (size_t)t, hidx.begin() + 6 * (size_t)t + 6);
+            cint.insert(cint.end(), hint.begin() + 3 * (size_t)t, hint.begin() + 3 * (size_t)t + 3);
+            cr.push_back(hr[t]);
+            cw.push_back(hw[t]);
+            cnode.push_back(t);
+        }
+    const long nn = (long)cr.size();
+    GDT_CHECK(nn > 0, GDT_E_ARG, "gdt_korobov_build: no lattice node survives the domain tests");
+    GDT_CHECK(nn < (1L << 29), GDT_E_ARG, "gdt_korobov_build: too many nodes");
+
+    // destination-sorted CSR by a stable counting sort: within a segment the
+    // entries follow (node, slot) order
+    const int np = h->n_pnt;
+    std::vector<int32_t> ptr(np + 1, 0);
+    auto slots = [&](long t) { return cr[t] == 0.0 ? 4 : 6; };
+    for (long t = 0; t < nn; t++)
+        for (int s = 0; s < slots(t); s++) ptr[cidx6[6 * t + s] + 1]++;
+    for (int v = 0; v < np; v++) ptr[v + 1] += ptr[v];
+    const long ne = ptr[np];
+    std::vector<uint32_t> ent(ne);
+    std::vector<int32_t> fillp(ptr.begin(), ptr.end() - 1);
+    for (long t = 0; t < nn; t++)
+        for (int s = 0; s < slots(t); s++) {
+            const uint32_t role = s < 2 ? 0u : (s < 4 ? 1u : 2u);
+            ent[fillp[cidx6[6 * t + s]]++] = ((uint32_t)t << 2) | role;
+        }
+
+    GDT_CHECK(h->kidx.ensure(sizeof(int32_t) * 6 * nn) == 0 && h->kr.ensure(sizeof(double) * nn) == 0 &&
+                  h->kw.ensure(sizeof(double) * nn) == 0 && h->knode.ensure(sizeof(uint32_t) * nn) == 0 &&
+                  h->kptr.ensure(sizeof(int32_t) * (np + 1)) == 0 && h->kent.ensure(sizeof(uint32_t) * ne) == 0,
+              GDT_E_NOMEM, "gdt_korobov_build: stencil allocation failed");
+    GDT_HIP(hipMemcpy(h->kidx.p, cidx6.data(), sizeof(int32_t) * 6 * nn, hipMemcpyHostToDevice));
+    GDT_HIP(hipMemcpy(h->kr.p, cr.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+    GDT_HIP(hipMemcpy(h->kw.p, cw.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+    GDT_HIP(hipMemcpy(h->knode.p, cnode.data(), sizeof(uint32_t) * nn, hipMemcpyHostToDevice));
+    GDT_HIP(hipMemcpy(h->kptr.p, ptr.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice));
+    GDT_HIP(hipMemcpy(h->kent.p, ent.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
+
+    // entry records for the fused kernel (needs 16-bit node indices and an
+    // LDS-resident f)
+    bool stream_ok = np <= 65535 && (size_t)np * sizeof(double) <= 150 * 1024;
+    for (long t = 0; t < nn && stream_ok; t++)
+        stream_ok = std::abs(cint[3 * t]) < 32768 && std::abs(cint[3 * t + 1]) < 32768 && cint[3 * t + 2] < 65536;
+    h->have_stream = false;
+    if (stream_ok) {
+        std::vector<uint32_t> e4(4 * (size_t)ne), e1(ne);
+        for (long e = 0; e < ne; e++) {
+            const uint32_t t = ent[e] >> 2, role = ent[e] & 3u;
+            const int32_t *o = &cidx6[6 * (size_t)t];
+            e4[4 * e + 0] = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+            e4[4 * e + 1] = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+            e4[4 * e + 2] = (uint32_t)o[4] | ((uint32_t)o[5] << 16);
+            e4[4 * e + 3] = ((uint32_t)cint[3 * t] & 0xFFFFu) | (((uint32_t)cint[3 * t + 1] & 0xFFFFu) << 16);
+            e1[e] = (uint32_t)cint[3 * t + 2] | (role << 16);
+        }
+        GDT_CHECK(h->ke4.ensure(sizeof(uint32_t) * 4 * ne) == 0 && h->ke1.ensure(sizeof(uint32_t) * ne) == 0,
+                  GDT_E_NOMEM, "gdt_korobov_build: entry-record allocation failed");
+        GDT_HIP(hipMemcpy(h->ke4.p, e4.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->ke1.p, e1.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
+        int rc2 = kor_fused_setup(h);
+        if (rc2) return rc2;
+        h->have_stream = true;
+    }
+    h->kM = M;
+    h->ka = a;
+    h->ks0 = s0;
+    h->kK = K;
+    h->nn = nn;
+    h->nent = ne;
+    h->have_kor = true;
+    if (h->graph_exec) {
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    return GDT_OK;
+}
+
+int gdt_korobov_info(gdt_handle h, uint32_t *M, uint32_t *a, long *nodes, long *entries, double *K)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_korobov_info: stencil not built");
+    if (M) *M = h->kM;
+    if (a) *a = h->ka;
+    if (nodes) *nodes = h->nn;
+    if (entries) *entries = h->nent;
+    if (K) *K = h->kK;
+    return GDT_OK;
+}
+
+int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32_t *node)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_korobov_export: stencil not built");
+    if (idx6) GDT_HIP(hipMemcpy(idx6, h->kidx.p, sizeof(int32_t) * 6 * h->nn, hipMemcpyDeviceToHost));
+    if (r) GDT_HIP(hipMemcpy(r, h->kr.p, sizeof(double) * h->nn, hipMemcpyDeviceToHost));
+    if (w) GDT_HIP(hipMemcpy(w, h->kw.p, sizeof(double) * h->nn, hipMemcpyDeviceToHost));
+    if (node) GDT_HIP(hipMemcpy(node, h->knode.p, sizeof(uint32_t) * h->nn, hipMemcpyDeviceToHost));
+    return GDT_OK;
+}
+
+int gdt_korobov_export_csr(gdt_handle h, int32_t *ptr, uint32_t *ent)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_korobov_export_csr: stencil not built");
+    if (ptr) GDT_HIP(hipMemcpy(ptr, h->kptr.p, sizeof(int32_t) * (h->n_pnt + 1), hipMemcpyDeviceToHost));
+    if (ent) GDT_HIP(hipMemcpy(ent, h->kent.p, sizeof(uint32_t) * h->nent, hipMemcpyDeviceToHost));
+    return GDT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// collision operator
+// ---------------------------------------------------------------------------
+
+static int to_cell_major(gdt_context *h, const double *src, int cells, int layout, DevBuf &tmp, const double **out)
+{
+    if (layout == GDT_LAYOUT_CELL_MAJOR) {
+        *out = src;
+        return GDT_OK;
+    }
+    GDT_CHECK(layout == GDT_LAYOUT_NODE_MAJOR, GDT_E_ARG, "unknown layout %d", layout);
+    GDT_CHECK(tmp.ensure(sizeof(double) * (size_t)cells * h->n_pnt) == 0, GDT_E_NOMEM, "layout scratch");
+    int rc = launch_transpose(h, src, tmp.as<double>(), h->n_pnt, cells);
+    *out = tmp.as<double>();
+    return rc;
+}
+
+// Matrix-method collide into cell-major compact outputs.
+static int collide_matrix_cm(gdt_context *h, const double *f_cm, double *nu, double *gain, int cells)
+{
+    const size_t nbox = (size_t)cells * h->n_box;
+    GDT_CHECK(h->s_fbox.ensure(sizeof(double) * nbox) == 0 && h->s_out1.ensure(sizeof(double) * nbox) == 0,
+              GDT_E_NOMEM, "collide scratch allocation failed");
+    int rc = launch_cmp_to_box(h, f_cm, h->s_fbox.as<double>(), cells, GDT_LAYOUT_CELL_MAJOR);
+    if (rc) return rc;
+    if (nu) {
+        if ((rc = launch_loss(h, h->s_fbox.as<double>(), h->s_out1.as<double>(), cells))) return rc;
+        if ((rc = launch_box_to_cmp(h, h->s_out1.as<double>(), nu, cells, GDT_LAYOUT_CELL_MAJOR))) return rc;
+    }
+    if (gain) {
+        if ((rc = launch_gain(h, h->s_fbox.as<double>(), h->s_out1.as<double>(), cells))) return rc;
+        if ((rc = launch_box_to_cmp(h, h->s_out1.as<double>(), gain, cells, GDT_LAYOUT_CELL_MAJOR))) return rc;
+    }
+    return GDT_OK;
+}
+
+static bool use_fused(gdt_context *h, unsigned flags) { return h->have_stream && !(flags & GDT_FLAG_TWO_PHASE); }
+
+static int ensure_mpart(gdt_context *h, int cells)
+{
+    const size_t need = sizeof(double) * 20 * (size_t)cells * kor_fused_blocks(h, cells);
+    GDT_CHECK(h->s_mpart.ensure(need) == 0, GDT_E_NOMEM, "moment-partial scratch allocation failed");
+    return GDT_OK;
+}
+
+// Korobov collide into cell-major loss / Q.  Fused path: one kernel, moment
+// partials left in s_mpart.  Two-phase path: node values then the
+// destination reduction, in chunks of cells that keep the node-value scratch
+// under 2 GiB.
+static int collide_korobov_cm(gdt_context *h, const double *f_cm, double *loss, double *Q, int cells,
+                              unsigned flags)
+{
+    if (use_fused(h, flags)) {
+        int rc = ensure_mpart(h, cells);
+        if (rc) return rc;
+        return launch_kor_fused(h, f_cm, loss, Q, h->s_mpart.as<double>(), cells);
+    }
+    const size_t per_cell = sizeof(double) * 4 * (size_t)h->nn;
+    int chunk = (int)std::max<size_t>(1, ((size_t)2 << 30) / per_cell);
+    chunk = std::min(chunk, cells);
+    GDT_CHECK(h->s_vals.ensure(per_cell * chunk) == 0, GDT_E_NOMEM, "node-value scratch allocation failed");
+    for (int c0 = 0; c0 < cells; c0 += chunk) {
+        const int cc = std::min(chunk, cells - c0);
+        const size_t o = (size_t)c0 * h->n_pnt;
+        int rc = launch_kor_values(h, f_cm + o, h->s_vals.as<double>(), cc);
+        if (rc) return rc;
+        rc = launch_kor_reduce(h, h->s_vals.as<double>(), loss ? loss + o : nullptr, Q + o, cc);
+        if (rc) return rc;
+    }
+    return GDT_OK;
+}
+
+int gdt_collide_matrix(gdt_handle h, const double *f, double *nu, double *gain, int cells, int layout)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_tables, GDT_E_STATE, "gdt_collide_matrix: tables not built");
+    GDT_CHECK(f && cells > 0, GDT_E_ARG, "gdt_collide_matrix: bad arguments");
+    const double *fcm;
+    int rc = to_cell_major(h, f, cells, layout, h->s_tmp, &fcm);
+    if (rc) return rc;
+    if (layout == GDT_LAYOUT_CELL_MAJOR) return collide_matrix_cm(h, fcm, nu, gain, cells);
+    const size_t nc = (size_t)cells * h->n_pnt;
+    GDT_CHECK(h->s_nu.ensure(sizeof(double) * nc) == 0 && h->s_out2.ensure(sizeof(double) * nc) == 0, GDT_E_NOMEM,
+              "collide scratch allocation failed");
+    rc = collide_matrix_cm(h, fcm, nu ? h->s_nu.as<double>() : nullptr, gain ? h->s_out2.as<double>() : nullptr, cells);
+    if (rc) return rc;
+    if (nu && (rc = launch_transpose(h, h->s_nu.as<double>(), nu, cells, h->n_pnt))) return rc;
+    if (gain && (rc = launch_transpose(h, h->s_out2.as<double>(), gain, cells, h->n_pnt))) return rc;
+    return GDT_OK;
+}
+
+int gdt_collide_korobov(gdt_handle h, const double *f, double *loss, double *Q, int cells, int layout)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_collide_korobov: stencil not built");
+    GDT_CHECK(f && Q && cells > 0, GDT_E_ARG, "gdt_collide_korobov: bad arguments");
+    const double *fcm;
+    int rc = to_cell_major(h, f, cells, layout, h->s_tmp, &fcm);
+    if (rc) return rc;
+    if (layout == GDT_LAYOUT_CELL_MAJOR) return collide_korobov_cm(h, fcm, loss, Q, cells, 0);
+    const size_t nc = (size_t)cells * h->n_pnt;
+    GDT_CHECK(h->s_loss.ensure(sizeof(double) * nc) == 0 && h->s_Q.ensure(sizeof(double) * nc) == 0, GDT_E_NOMEM,
+              "collide scratch allocation failed");
+    rc = collide_korobov_cm(h, fcm, h->s_loss.as<double>(), h->s_Q.as<double>(), cells, 0);
+    if (rc) return rc;
+    if (loss && (rc = launch_transpose(h, h->s_loss.as<double>(), loss, cells, h->n_pnt))) return rc;
+    return launch_transpose(h, h->s_Q.as<double>(), Q, cells, h->n_pnt);
+}
+
+int gdt_correct(gdt_handle h, int method, const double *f, double *nu, const double *gain, double *Q, double *X,
+                int cells, int layout)
+{
+    H_CHECK(h);
+    GDT_CHECK(f && nu && gain && Q && cells > 0, GDT_E_ARG, "gdt_correct: bad arguments");
+    GDT_CHECK(layout == GDT_LAYOUT_CELL_MAJOR, GDT_E_ARG, "gdt_correct: cell-major layout only");
+    GDT_CHECK(method == GDT_METHOD_MATRIX || method == GDT_METHOD_KOROBOV, GDT_E_ARG, "gdt_correct: method %d",
+              method);
+    if (method == GDT_METHOD_KOROBOV) return launch_correct(h, method, f, nu, gain, nullptr, Q, X, cells, 0.0, nullptr);
+    return launch_correct(h, method, f, nu, gain, nu, Q, X, cells, 0.0, nullptr);
+}
+
+int gdt_evaluate(gdt_handle h, int method, const double *f, double *nu, double *gain, double *Q, int cells, int layout,
+                 unsigned flags)
+{
+    H_CHECK(h);
+    GDT_CHECK(f && cells > 0, GDT_E_ARG, "gdt_evaluate: bad arguments");
+    const bool timing = (flags & GDT_FLAG_TIME) != 0;
+    h->timing = timing;
+    const size_t nc = (size_t)cells * h->n_pnt;
+    int rc = GDT_OK;
+    const double *fcm;
+    if ((rc = to_cell_major(h, f, cells, layout, h->s_tmp, &fcm))) goto done;
+    GDT_CHECK(h->s_nu.ensure(sizeof(double) * nc) == 0 && h->s_out2.ensure(sizeof(double) * nc) == 0 &&
+                  h->s_Q.ensure(sizeof(double) * nc) == 0 && h->s_loss.ensure(sizeof(double) * nc) == 0,
+              GDT_E_NOMEM, "gdt_evaluate: scratch allocation failed");
+    {
+        double *nu_c = h->s_nu.as<double>(), *g_c = h->s_out2.as<double>(), *q_c = h->s_Q.as<double>();
+        if (method == GDT_METHOD_MATRIX) {
+            GDT_CHECK(h->have_tables, GDT_E_STATE, "gdt_evaluate: tables not built");
+            if ((rc = collide_matrix_cm(h, fcm, nu_c, g_c, cells))) goto done;
+            if (flags & GDT_FLAG_CORRECT) {
+                if ((rc = launch_correct(h, method, fcm, nu_c, g_c, nu_c, q_c, nullptr, cells, 0.0, nullptr))) goto done;
+            } else {
+                if ((rc = launch_correct(h, -1, fcm, nu_c, g_c, nu_c, q_c, nullptr, cells, 0.0, nullptr))) goto done;
+            }
+        } else if (method == GDT_METHOD_KOROBOV) {
+            GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_evaluate: stencil not built");
+            double *L = h->s_loss.as<double>();
+            if ((rc = collide_korobov_cm(h, fcm, L, g_c, cells, flags))) goto done;
+            // g_c holds the raw Q; correct into q_c (nu_c = corrected loss frequency)
+            if (use_fused(h, flags)) {
+                if ((rc = launch_kor_apply(h, (flags & GDT_FLAG_CORRECT) != 0, h->s_mpart.as<double>(), fcm, L, g_c,
+                                           q_c, nu_c, nullptr, cells, 0.0, nullptr)))
+                    goto done;
+            } else {
+                const int m = (flags & GDT_FLAG_CORRECT) ? GDT_METHOD_KOROBOV : -2;
+                if ((rc = launch_correct(h, m, fcm, L, g_c, nu_c, q_c, nullptr, cells, 0.0, nullptr))) goto done;
+            }
+            // gain = Q + f nu_c
+            if ((rc = launch_correct(h, -3, fcm, nu_c, q_c, nullptr, g_c, nullptr, cells, 0.0, nullptr))) goto done;
+        } else {
+            gdt_set_error(GDT_E_ARG, "gdt_evaluate: method %d", method);
+            rc = GDT_E_ARG;
+            goto done;
+        }
+        double *outs[3] = {nu, gain, Q};
+        double *srcs[3] = {nu_c, g_c, q_c};
+        for (int q = 0; q < 3; q++) {
+            if (!outs[q]) continue;
+            if (layout == GDT_LAYOUT_CELL_MAJOR)
+                GDT_HIP(hipMemcpyAsync(outs[q], srcs[q], sizeof(double) * nc, hipMemcpyDeviceToDevice, h->stream));
+            else if ((rc = launch_transpose(h, srcs[q], outs[q], cells, h->n_pnt)))
+                goto done;
+        }
+    }
+done:
+    h->timing = false;
+    return rc;
+}
+
+int gdt_relax(gdt_handle h, double *f, const double *nu, const double *Q, int cells, double dt, double courant,
+              double *time, int layout, unsigned flags)
+{
+    H_CHECK(h);
+    GDT_CHECK(f && Q && cells > 0, GDT_E_ARG, "gdt_relax: bad arguments");
+    GDT_CHECK(layout == GDT_LAYOUT_CELL_MAJOR || layout == GDT_LAYOUT_NODE_MAJOR, GDT_E_ARG, "gdt_relax: layout");
+    const long count = (long)cells * h->n_pnt;  // elementwise: layout-independent
+    if (flags & GDT_FLAG_COURANT) {
+        GDT_CHECK(nu, GDT_E_ARG, "gdt_relax: courant step needs nu");
+        GDT_CHECK(h->s_dt.ensure(sizeof(double)) == 0, GDT_E_NOMEM, "gdt_relax: scratch");
+        int rc = launch_courant_dt(h, nu, count, courant, h->s_dt.as<double>());
+        if (rc) return rc;
+        return launch_euler(h, f, Q, count, 0.0, h->s_dt.as<double>(), time);
+    }
+    return launch_euler(h, f, Q, count, dt, nullptr, time);
+}
+
+// One explicit step of cell-major f (no host synchronisation).
+static int one_step(gdt_context *h, int method, double *f, int cells, double dt, unsigned flags)
+{
+    int rc;
+    const bool corr = (flags & GDT_FLAG_CORRECT) != 0;
+    if (method == GDT_METHOD_KOROBOV) {
+        double *L = h->s_loss.as<double>(), *Q0 = h->s_out2.as<double>(), *Q = h->s_Q.as<double>();
+        if ((rc = collide_korobov_cm(h, f, L, Q0, cells, flags))) return rc;
+        if (use_fused(h, flags))
+            return launch_kor_apply(h, corr, h->s_mpart.as<double>(), f, L, Q0, Q, nullptr, nullptr, cells, dt, f);
+        return launch_correct(h, corr ? GDT_METHOD_KOROBOV : -2, f, L, Q0, nullptr, Q, nullptr, cells, dt, f);
+    }
+    double *nu = h->s_nu.as<double>(), *g = h->s_out2.as<double>(), *Q = h->s_Q.as<double>();
+    if ((rc = collide_matrix_cm(h, f, nu, g, cells))) return rc;
+    return launch_correct(h, corr ? GDT_METHOD_MATRIX : -1, f, nu, g, nu, Q, nullptr, cells, dt, f);
+}
+
+int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int steps, unsigned flags)
+{
+    H_CHECK(h);
+    GDT_CHECK(f && cells > 0 && steps >= 0, GDT_E_ARG, "gdt_run_steps: bad arguments");
+    GDT_CHECK(method == GDT_METHOD_MATRIX || method == GDT_METHOD_KOROBOV, GDT_E_ARG, "gdt_run_steps: method");
+    if (method == GDT_METHOD_KOROBOV) GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_run_steps: stencil not built");
+    if (method == GDT_METHOD_MATRIX) GDT_CHECK(h->have_tables, GDT_E_STATE, "gdt_run_steps: tables not built");
+    const size_t nc = (size_t)cells * h->n_pnt;
+    GDT_CHECK(h->s_nu.ensure(sizeof(double) * nc) == 0 && h->s_out2.ensure(sizeof(double) * nc) == 0 &&
+                  h->s_Q.ensure(sizeof(double) * nc) == 0 && h->s_loss.ensure(sizeof(double) * nc) == 0,
+              GDT_E_NOMEM, "gdt_run_steps: scratch allocation failed");
+    if (method == GDT_METHOD_MATRIX) {
+        const size_t nbox = (size_t)cells * h->n_box;
+        GDT_CHECK(h->s_fbox.ensure(sizeof(double) * nbox) == 0 && h->s_out1.ensure(sizeof(double) * nbox) == 0,
+                  GDT_E_NOMEM, "gdt_run_steps: scratch allocation failed");
+    } else if (use_fused(h, flags)) {
+        int rc0 = ensure_mpart(h, cells);
+        if (rc0) return rc0;
+    } else {
+        const size_t per_cell = sizeof(double) * 4 * (size_t)h->nn;
+        const int chunk = std::min<int>(cells, (int)std::max<size_t>(1, ((size_t)2 << 30) / per_cell));
+        GDT_CHECK(h->s_vals.ensure(per_cell * chunk) == 0, GDT_E_NOMEM, "gdt_run_steps: scratch allocation failed");
+    }
+    const bool graph = (flags & GDT_FLAG_GRAPH) && !(flags & GDT_FLAG_TIME);
+    h->timing = (flags & GDT_FLAG_TIME) != 0;
+    int rc = GDT_OK;
+    if (!graph) {
+        for (int s = 0; s < steps && rc == GDT_OK; s++) rc = one_step(h, method, f, cells, dt, flags);
+        h->timing = false;
+        return rc;
+    }
+    h->timing = false;
+    if (steps == 0) return GDT_OK;
+    // one graph holds `chunk` consecutive steps (a divisor of steps when one
+    // <= 64 exists), so replays are back to back
+    int chunk = 1;
+    for (int c = std::min(steps, 64); c >= 1; c--)
+        if (steps % c == 0) {
+            chunk = c;
+            break;
+        }
+    if (chunk < 8 && steps > 64) chunk = 64;
+    const unsigned key = flags & (GDT_FLAG_CORRECT | GDT_FLAG_TWO_PHASE);
+    if (!h->graph_exec || h->g_method != method || h->g_cells != cells || h->g_dt != dt || h->g_flags != key ||
+        h->g_f != f || h->g_chunk != chunk) {
+        if (h->graph_exec) {
+            (void)hipGraphExecDestroy(h->graph_exec);
+            h->graph_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        GDT_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int s2 = 0; s2 < chunk && rc == GDT_OK; s2++) rc = one_step(h, method, f, cells, dt, flags);
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (rc != GDT_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        GDT_HIP(e);
+        e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        GDT_HIP(e);
+        h->g_method = method;
+        h->g_cells = cells;
+        h->g_dt = dt;
+        h->g_flags = key;
+        h->g_f = f;
+        h->g_chunk = chunk;
+    }
+    const int full = steps / chunk, rem = steps % chunk;
+    for (int s = 0; s < full; s++) GDT_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+    for (int s = 0; s < rem && rc == GDT_OK; s++) rc = one_step(h, method, f, cells, dt, flags);
+    return rc;
+}
+
+int gdt_kernel_stats(gdt_handle h, const char *name, double *total_ms, long *count)
+{
+    H_CHECK(h);
+    GDT_CHECK(name, GDT_E_ARG, "gdt_kernel_stats: name is null");
+    auto it = h->stats.find(name);
+    if (total_ms) *total_ms = it == h->stats.end() ? 0.0 : it->second.ms;
+    if (count) *count = it == h->stats.end() ? 0 : it->second.count;
+    return GDT_OK;
+}
+
+int gdt_kernel_stats_reset(gdt_handle h)
+{
+    H_CHECK(h);
+    h->stats.clear();
+    return GDT_OK;
+}
+
+int gdt_transpose(gdt_handle h, const double *src, double *dst, int rows, int cols)
+{
+    H_CHECK(h);
+    GDT_CHECK(src && dst && rows > 0 && cols > 0, GDT_E_ARG, "gdt_transpose: bad arguments");
+    return launch_transpose(h, src, dst, rows, cols);
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Internal state and kernel launchers of libgdt.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gdt.h"
+
+void gdt_set_error(int code, const char *fmt, ...);
+
+#define GDT_HIP(call)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            gdt_set_error(GDT_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return GDT_E_HIP;                                                                           \
+        }                                                                                               \
+    } while (0)
+
+#define GDT_CHECK(cond, code, ...)                \
+    do {                                          \
+        if (!(cond)) {                            \
+            gdt_set_error((code), __VA_ARGS__);   \
+            return (code);                        \
+        }                                         \
+    } while (0)
+
+// Device buffer owned by a context.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);  // grow (contents not kept)
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct KernelStat {
+    double ms = 0.0;
+    long count = 0;
+};
+
+struct gdt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // grid
+    int n[3] = {0, 0, 0};
+    double vmin[3], vmax[3], R = 0.0, d[3], d3v = 0.0;
+    int n_box = 0, n_pnt = 0;
+    std::vector<int> h_box2c, h_cidx;
+    std::vector<double> h_xi;
+    DevBuf box2c, cidx, xi, cmp2box;
+
+    // analytic matrix tables
+    int table_mode = -1;
+    bool have_tables = false;
+    long n_a = 0, n_gent = 0;
+    DevBuf tb, ta, gptr, gbox, gval;
+
+    // Korobov stencil
+    bool have_kor = false;
+    uint32_t kM = 0, ka = 0;
+    double ks0 = 0.0, kK = 0.0;
+    long nn = 0, nent = 0;
+    DevBuf kidx, kr, kw, knode, kptr, kent;
+    // fused path: per-entry records in CSR order (uint4 node pairs + energy
+    // numerator/denominator, uint32 D2 | role << 16)
+    bool have_stream = false;
+    DevBuf ke4, ke1;
+    int fused_blocks = 0;  // blocks per cell of the fused kernel (0 = not chosen)
+
+    // scratch
+    DevBuf s_fbox, s_out1, s_out2, s_vals, s_loss, s_Q, s_nu, s_tmp, s_dt, s_mpart;
+
+    // step graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    int g_method = -1, g_cells = 0, g_chunk = 0;
+    double g_dt = 0.0;
+    unsigned g_flags = 0;
+    double *g_f = nullptr;
+
+    // timing
+    bool timing = false;
+    std::map<std::string, KernelStat> stats;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// Records a kernel launch with hipEvents on the context stream when timing is on.
+struct KernelTimer {
+    gdt_context *h;
+    const char *name;
+    KernelTimer(gdt_context *h_, const char *name_);
+    ~KernelTimer();
+};
+
+// ---- setup kernels (setup_kernels.hip) -----------------------------------
+int launch_build_tables(gdt_context *h, int mode);
+int launch_korobov_nodes(gdt_context *h, uint32_t M, uint32_t a, double K, int32_t *d_idx6, double *d_r,
+                         double *d_w, int32_t *d_ints, uint8_t *d_valid);
+uint32_t host_korobov_pick(uint32_t M, uint64_t seed, int ncand, double *p2);
+
+// ---- collision kernels (collide_kernels.hip) ------------------------------
+int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells);
+int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells);
+int launch_cmp_to_box(gdt_context *h, const double *f_cmp, double *f_box, int cells, int layout);
+int launch_box_to_cmp(gdt_context *h, const double *f_box, double *f_cmp, int cells, int layout);
+int launch_kor_values(gdt_context *h, const double *f, double *vals, int cells);
+int launch_kor_reduce(gdt_context *h, const double *vals, double *loss, double *Q, int cells);
+// fused path: Q, loss and per-block moment partials in one pass; then the
+// correction (+ optional Euler step) from the partials
+int kor_fused_blocks(gdt_context *h, int cells);
+int kor_fused_setup(gdt_context *h);
+int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells);
+int launch_kor_apply(gdt_context *h, bool correct, const double *mpart, const double *f, const double *loss,
+                     const double *Qin, double *Qout, double *nu_out, double *X, int cells, double dt, double *f_upd);
+int launch_correct(gdt_context *h, int method, const double *f, const double *nu, const double *gain,
+                   double *nu_out, double *Q, double *X, int cells, double dt, double *f_upd);
+int launch_transpose(gdt_context *h, const double *src, double *dst, int rows, int cols);
+int launch_courant_dt(gdt_context *h, const double *nu, long count, double courant, double *dt_out);
+int launch_euler(gdt_context *h, double *f, const double *Q, long count, double dt, const double *dt_dev,
+                 double *time);
+int launch_basis(gdt_context *h, double *dst);
+int launch_fill(gdt_context *h, double *dst, long count, double value);

@@ -1,0 +1,162 @@
+/*
+ * gdt.h -- C ABI of the MI355X collision-integral library (libgdt.so).
+ *
+ * Plain C: opaque handle, plain pointers and sizes, int status codes.  Every
+ * velocity-space array argument is DEVICE memory of the handle's GPU, FP64,
+ * in the compact sphere-cut node order of the handle's grid (i-j-k order of
+ * the kept nodes, velocity_grid.cu:114-126), laid out [cell][node] unless a
+ * `layout` argument says GDT_LAYOUT_NODE_MAJOR ([node][cell], the layout of
+ * the reference's arrays, determ.cu:116).  Calls are asynchronous on the
+ * handle's stream unless stated; gdt_synchronize() waits.
+ *
+ * Status: 0 = ok, < 0 = error; gdt_last_error() returns the message of the
+ * calling thread's last failure.
+ *
+ * Each entry point names the reference interface it replaces.  The C++
+ * drop-in layer (include/gpu_determ/*.h, libgpu_determ.so) keeps the
+ * reference's own names and signatures and is built on these calls.
+ */
+#ifndef GDT_H
+#define GDT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GDT_OK 0
+#define GDT_E_ARG -1      /* invalid argument (null, size, unsupported grid) */
+#define GDT_E_HIP -2      /* HIP runtime failure                             */
+#define GDT_E_STATE -3    /* required tables / stencil not built             */
+#define GDT_E_NOMEM -4    /* device allocation failed                        */
+#define GDT_E_NODEVICE -5 /* no HIP device                                   */
+
+#define GDT_LAYOUT_CELL_MAJOR 0 /* [cell][node] */
+#define GDT_LAYOUT_NODE_MAJOR 1 /* [node][cell], the reference's layout */
+
+#define GDT_TABLES_FP64 0      /* FP64 spacing and relative speed          */
+#define GDT_TABLES_REFERENCE 1 /* FP32 spacing / g / stored values, as the
+                                  reference's store_matrix_elements        */
+
+#define GDT_METHOD_MATRIX 0  /* analytic collision matrix (the reference's) */
+#define GDT_METHOD_KOROBOV 1 /* Korobov lattice + two-point projection      */
+
+#define GDT_FLAG_CORRECT 1u    /* apply the conservation correction         */
+#define GDT_FLAG_GRAPH 2u      /* replay the step sequence as a hipGraph    */
+#define GDT_FLAG_TIME 4u       /* time each kernel with hipEvents           */
+#define GDT_FLAG_COURANT 8u    /* dt = courant / max nu (relax_f semantics) */
+#define GDT_FLAG_TWO_PHASE 16u /* Korobov: node-value + reduce kernels instead
+                                  of the fused LDS kernel                   */
+
+typedef struct gdt_context *gdt_handle;
+
+/* Velocity grid: n[3] nodes on [vmin, vmax] per axis, sphere cut by R
+ * (R <= 0: keep all).  Replaces VelocityGrid::init (velocity_grid.cu:73-126,
+ * velocity_grid.hpp:35-54). */
+typedef struct gdt_grid_desc {
+    int n[3];
+    double vmin[3];
+    double vmax[3];
+    double R;
+} gdt_grid_desc;
+
+/* ---- errors, device plumbing ------------------------------------------ */
+const char *gdt_last_error(void);
+int gdt_version(void);
+int gdt_device_count(int *count);
+int gdt_malloc(void **dptr, size_t bytes);
+int gdt_free(void *dptr);
+int gdt_memcpy_htod(void *dst, const void *src, size_t bytes);
+int gdt_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int gdt_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int gdt_memset(void *dptr, int value, size_t bytes);
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates the grid on `device`: node lists, centres and the moment basis.
+ * Replaces VelocityGrid::device() / DeviceVelocityGrid (velocity_grid.cu:
+ * 21-48) and init_correction_array (velocity.cu:53-65). */
+int gdt_create(gdt_handle *h, const gdt_grid_desc *grid, int device);
+int gdt_destroy(gdt_handle h);
+int gdt_synchronize(gdt_handle h);
+/* Grid facts: n_pnt kept nodes, n_box box nodes, spacing d[3], d3v. */
+int gdt_grid_info(gdt_handle h, int *n_pnt, int *n_box, double d[3], double *d3v);
+/* Host copies: box coordinates [n_pnt][3] and centres [n_pnt][3]. */
+int gdt_grid_nodes(gdt_handle h, int *cidx, double *xi);
+/* 14 moment-basis planes over the box (xi_x, xi_y, xi_z, xi^2, xi_x^2, ...,
+ * xi^4), FP64 device array of 14*n_box: fill_correction_array
+ * (velocity.cu:17-50).  dst is device memory. */
+int gdt_basis(gdt_handle h, double *dst);
+
+/* ---- analytic collision matrix (init_matrices, compute_matrix.cu:564-600) */
+/* Builds b[n_box] and a[n_box*n_box/8] (reference index order) on the GPU in
+ * FP64, and the per-relative-velocity lists of nonzero boxes.  mode is
+ * GDT_TABLES_FP64 or GDT_TABLES_REFERENCE.  Requires even n on every axis
+ * and a box of at most 24^3 nodes. */
+int gdt_build_tables(gdt_handle h, int mode);
+/* Device pointers to the FP64 tables owned by the handle. */
+int gdt_tables(gdt_handle h, const double **b, const double **a, long *n_a, long *nnz);
+/* Replace the handle's tables by caller-provided ones (device FP64, same
+ * layout), e.g. tables imported from FP32 reference arrays. */
+int gdt_set_tables(gdt_handle h, const double *b, const double *a);
+
+/* ---- Korobov lattice stencil (no reference counterpart) ---------------- */
+/* Picks a generator from seed (a = 0) or uses `a`, generates the lattice on
+ * the GPU, keeps the nodes whose six projection nodes lie in the grid, and
+ * builds the destination-sorted CSR.  s0 is the differential cross-section
+ * per steradian (1: the reference's 4 pi g total; 0.25: unit diameter).
+ * Requires a cubic grid with equal spacing and M a power of two <= 2^26. */
+int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, double s0);
+int gdt_korobov_info(gdt_handle h, uint32_t *M, uint32_t *a, long *nodes, long *entries, double *K);
+/* Host copies of the stencil: idx6[nodes][6] (alpha, beta, lambda, mu,
+ * lambda+s, mu-s), r[nodes], w[nodes], node[nodes] (lattice number). */
+int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32_t *node);
+/* Host copy of the CSR: ptr[n_pnt+1], ent[entries] = node*4 + role. */
+int gdt_korobov_export_csr(gdt_handle h, int32_t *ptr, uint32_t *ent);
+/* Generator choice alone (host): same rule as gdt_korobov_build. */
+uint32_t gdt_korobov_pick(uint32_t M, uint64_t seed, int ncand, double *p2);
+
+/* ---- collision operator ------------------------------------------------- */
+/* Matrix method: nu (loss frequency, evolve_direct_colisions,
+ * collision_integral.cu:32-72) and gain (evolve_inverse_colisions, :75-188,
+ * all 512 boxes, intended output slot) for `cells` cells.  Either output may
+ * be NULL. */
+int gdt_collide_matrix(gdt_handle h, const double *f, double *nu, double *gain, int cells, int layout);
+/* Korobov method: loss = f nu and Q (uncorrected) for `cells` cells. */
+int gdt_collide_korobov(gdt_handle h, const double *f, double *loss, double *Q, int cells, int layout);
+/* Conservation correction (before_correction / linear_solver /
+ * correct_frequency, collision_integral.cu:217-526), for every cell:
+ * method MATRIX: in f, nu, gain -> nu_corr (may alias nu), Q;
+ * method KOROBOV: in loss (passed as nu), Q (in/out, passed as gain == Q).
+ * X (5 per cell, device) may be NULL. */
+int gdt_correct(gdt_handle h, int method, const double *f, double *nu, const double *gain, double *Q,
+                double *X, int cells, int layout);
+/* One full evaluation = evolve_colisions (collision_integral.cu:550-773):
+ * collide, then correct if flags & GDT_FLAG_CORRECT.  Outputs: nu (corrected
+ * loss frequency; KOROBOV: loss/f), gain (KOROBOV: Q + loss), Q. */
+int gdt_evaluate(gdt_handle h, int method, const double *f, double *nu, double *gain, double *Q, int cells,
+                 int layout, unsigned flags);
+/* relax_f (collision_integral.cu:835-880): f <- f + dt Q for all cells, with
+ * dt from courant (flags & GDT_FLAG_COURANT: dt = min courant/nu) or `dt`.
+ * time (device double, may be NULL) += dt. */
+int gdt_relax(gdt_handle h, double *f, const double *nu, const double *Q, int cells, double dt, double courant,
+              double *time, int layout, unsigned flags);
+/* `steps` explicit steps df/dt = Q(f,f) of `cells` cells (cell-major),
+ * each: collide, correct (if flags & GDT_FLAG_CORRECT), f += dt Q.  With
+ * GDT_FLAG_GRAPH the step is captured once as a hipGraph and replayed. */
+int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int steps, unsigned flags);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Kernel timing under GDT_FLAG_TIME: total milliseconds and launch count of
+ * the kernel `name` since the last reset. */
+int gdt_kernel_stats(gdt_handle h, const char *name, double *total_ms, long *count);
+int gdt_kernel_stats_reset(gdt_handle h);
+/* Layout transposes (transpose_float, collision_integral.cu:528-547):
+ * [rows][cols] -> [cols][rows], device FP64. */
+int gdt_transpose(gdt_handle h, const double *src, double *dst, int rows, int cols);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

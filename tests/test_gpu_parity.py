@@ -1,0 +1,291 @@
+"""GPU parity of libgdt.so (through its C ABI) against the FP64 oracle.
+
+Bars: integer and selection work bit-exact; FP64 values <= 1e-12 relative
+(L2 over the array unless stated).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gpu_determ_amd as G
+import oracle as O
+from helpers import maxwellian, mixture, moments, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def ctx_for(g):
+    return G.Context(g.n, g.vmin, g.vmax, g.R)
+
+
+@pytest.mark.parametrize("n,mode", [((8, 8, 8), 0), ((16, 16, 16), 0), ((16, 16, 16), 1)])
+def test_tables_match_oracle(n, mode):
+    say("tables", n, mode)
+    g = O.Grid(n, (-5, -5, -5), (5, 5, 5), -1.0)
+    c = ctx_for(g)
+    c.build_tables(mode)
+    b, a, nnz = c.tables()
+    ob, oa = O.build_tables(g, mode)
+    # selection (A < ERR) is integer work: identical zero pattern
+    assert np.array_equal(a != 0.0, oa != 0.0)
+    assert nnz == np.count_nonzero(oa)
+    if mode == 0:
+        # a = area * 2 g d3v; the area is a difference of O(1) antiderivatives,
+        # so it agrees to an absolute 1e-14 (b = 4 pi g d3v gives the scale)
+        nu_, nv_, nw_ = n
+        scale = np.broadcast_to((ob / (2 * np.pi)).reshape(1, nu_, 1, 1, nv_, nw_),
+                                (nu_ // 2, nu_, nv_ // 2, nw_ // 2, nv_, nw_)).ravel()
+        assert np.all(np.abs(a - oa) <= 1e-14 * scale)
+        assert np.max(np.abs(b - ob)) <= 1e-15 * np.max(np.abs(ob))
+    else:
+        # FP32-rounded values: equal, or one float ulp apart where the FP64
+        # area sits on a float rounding boundary
+        ulp = np.spacing(np.abs(oa).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(a - oa) <= ulp)
+        assert np.array_equal(b, ob)
+
+
+def test_loss_gain_8cube():
+    say("loss/gain 8^3")
+    g = O.Grid((8, 8, 8), (-4, -4, -4), (4, 4, 4), -1.0)
+    c = ctx_for(g)
+    c.build_tables(0)
+    ob, oa = O.build_tables(g, 0)
+    rng = np.random.default_rng(1)
+    f = np.stack([mixture(g.xi, rng) for _ in range(3)])
+    fd = G.DeviceArray.from_host(f)
+    nu, gain = c.collide_matrix(fd, 3)
+    nu, gain = nu.get(), gain.get()
+    onu = O.loss(g, ob, g.to_box(f))
+    og = O.gain(g, oa, g.to_box(f))
+    assert rel_l2(nu, g.to_cmp(onu)) < 1e-13
+    assert rel_l2(gain, g.to_cmp(og)) < 1e-13
+
+
+def test_matrix_sphere_cut_16():
+    say("matrix sphere-cut 16^3")
+    g = O.Grid((16, 16, 16), (-4.8, -4.8, -4.8), (4.8, 4.8, 4.8), 4.8)
+    c = ctx_for(g)
+    c.build_tables(0)
+    ob, oa = O.build_tables(g, 0)
+    rng = np.random.default_rng(2)
+    f = mixture(g.xi, rng)[None]
+    fd = G.DeviceArray.from_host(f)
+    nu, gain = c.collide_matrix(fd, 1)
+    nu, gain = nu.get(), gain.get()
+    fb = g.to_box(f)
+    onu = g.to_cmp(O.loss(g, ob, fb))
+    dest = g.box_index()[::7].astype(np.int32)
+    og = O.gain(g, oa, fb, dest)[0]
+    assert rel_l2(nu, onu) < 1e-13
+    assert rel_l2(gain[0, ::7], og) < 1e-13
+
+
+def test_reference_golden_through_gpu(ref_grid):
+    """The reference's own case (16^3, FP32 tables, its initial f) through
+    the GPU path reproduces the golden rows of true_results.out."""
+    say("golden 16^3")
+    g = ref_grid
+    rows = np.genfromtxt(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                                    "reference_rows.tsv"), names=True, delimiter="\t")
+    c = ctx_for(g)
+    c.build_tables(G.TABLES_REFERENCE)
+    f = O.init_f_reference(g)
+    cells = 4
+    fd = G.DeviceArray.from_host(np.tile(f, (cells, 1)))
+    nu, gain = c.collide_matrix(fd, cells)
+    nu, gain = nu.get(), gain.get()
+    assert np.array_equal(nu[0], nu[cells - 1]) and np.array_equal(gain[0], gain[1])
+    slot = rows["slot"].astype(int)
+    keep = rows["cell0"] == 0
+    dc = nu[0][slot]
+    assert np.max(np.abs(dc[keep] - rows["DC"][keep]) / rows["DC"][keep]) < 5e-6
+    ic = gain[0][slot]
+    relic = np.abs(ic - rows["IC"]) / rows["IC"]
+    assert np.max(relic[rows["race"] == 0]) < 1e-4
+    assert np.max(relic) < 2e-4
+
+
+def test_correction_conserves():
+    say("correction")
+    g = O.Grid((16, 16, 16), (-4.8, -4.8, -4.8), (4.8, 4.8, 4.8), 4.8)
+    c = ctx_for(g)
+    c.build_tables(0)
+    rng = np.random.default_rng(3)
+    cells = 3
+    f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
+    fd = G.DeviceArray.from_host(f)
+    nu, gain, Q = c.evaluate(G.METHOD_MATRIX, fd, cells, G.FLAG_CORRECT)
+    nu, gain, Q = nu.get(), gain.get(), Q.get()
+    nu0, g0 = c.collide_matrix(fd, cells)
+    nuc, Qo, X, bad = O.correct(g.xi, f, nu0.get(), g0.get(), cells)
+    assert bad == 0
+    assert rel_l2(Q, Qo) < 1e-12
+    assert rel_l2(nu, nuc) < 1e-12
+    m = moments(g.xi, Q, g.d3v)
+    scale = moments(g.xi, np.abs(g0.get()), g.d3v)
+    assert np.max(np.abs(m) / np.abs(scale)) < 1e-13
+
+
+@pytest.mark.parametrize("n,M,s0", [((16, 16, 16), 1 << 16, 1.0), ((20, 20, 20), 1 << 18, 0.25)])
+def test_korobov_stencil_bitexact(n, M, s0):
+    say("korobov stencil", n, M)
+    vm = 4.8
+    g = O.Grid(n, (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    a, _ = O.korobov_pick(M, 11, 8)
+    c.korobov_build(M, seed=11, a=0, s0=s0)
+    info = c.korobov_info()
+    assert info["a"] == a
+    assert abs(info["K"] - O.korobov_K(g, M, s0)) <= 1e-15 * info["K"]
+    idx6, r, w, node = c.korobov_export()
+    L = O.lib()
+    cnt = L.orc_korobov_stencil(O._i3(g.n), g.box2c, g.n_pnt, M, a, info["K"], None, None, None, None, 0)
+    oi = np.empty(6 * cnt, np.int32)
+    orr = np.empty(cnt)
+    ow = np.empty(cnt)
+    on = np.empty(cnt, np.uint32)
+    L.orc_korobov_stencil(O._i3(g.n), g.box2c, g.n_pnt, M, a, info["K"], oi, orr, ow, on, cnt)
+    assert info["nodes"] == cnt
+    assert np.array_equal(node, on)
+    assert np.array_equal(idx6.ravel(), oi)
+    assert np.array_equal(r, orr)
+    assert np.array_equal(w, ow)
+    ptr, ent = c.korobov_export_csr()
+    assert ptr[-1] == info["entries"] == 4 * cnt + 2 * np.count_nonzero(r)
+    # destination-sorted and, within a segment, in node order
+    nodes = (ent >> 2).astype(np.int64)
+    for v in range(0, g.n_pnt, 97):
+        seg = nodes[ptr[v]:ptr[v + 1]]
+        assert np.all(np.diff(seg) >= 0)
+
+
+@pytest.mark.parametrize("cells", [1, 3])
+def test_korobov_collide_matches_oracle(cells):
+    say("korobov collide", cells)
+    vm = 4.8
+    g = O.Grid((20, 20, 20), (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    M = 1 << 18
+    c.korobov_build(M, seed=5, s0=0.25)
+    info = c.korobov_info()
+    st = O.korobov_stencil(g, M, info["a"], 0.25)
+    rng = np.random.default_rng(4)
+    f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
+    fd = G.DeviceArray.from_host(f)
+    loss, Q = c.collide_korobov(fd, cells)
+    ol, oq = O.korobov_collide(g, st, f)
+    assert rel_l2(loss.get(), ol) < 1e-12
+    assert rel_l2(Q.get(), oq) < 1e-12
+    # node-major layout gives the same numbers
+    fT = G.DeviceArray.from_host(np.ascontiguousarray(f.T))
+    l2, q2 = c.collide_korobov(fT, cells, G.LAYOUT_NODE_MAJOR)
+    q2h = q2.get().reshape(g.n_pnt, cells).T
+    assert np.array_equal(q2h, Q.get())
+    assert np.array_equal(l2.get().reshape(g.n_pnt, cells).T, loss.get())
+
+
+def test_korobov_maxwellian_and_conservation():
+    say("korobov maxwellian")
+    vm = 4.8
+    g = O.Grid((20, 20, 20), (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    c.korobov_build(1 << 18, seed=5, s0=0.25)
+    fm = maxwellian(g.xi, 1.0, 1.0, (0.3, -0.2, 0.1))[None]
+    fd = G.DeviceArray.from_host(fm)
+    loss, Q = c.collide_korobov(fd, 1)
+    assert np.max(np.abs(Q.get())) < 1e-13 * np.max(loss.get())
+    rng = np.random.default_rng(9)
+    f = mixture(g.xi, rng)[None]
+    fd = G.DeviceArray.from_host(f)
+    nu, gain, Qc = c.evaluate(G.METHOD_KOROBOV, fd, 1, G.FLAG_CORRECT)
+    m = moments(g.xi, Qc.get(), g.d3v)
+    scale = moments(g.xi, np.abs(gain.get()), g.d3v)
+    assert np.max(np.abs(m) / np.abs(scale)) < 1e-13
+    lo, q0 = c.collide_korobov(fd, 1)
+    oq, X, bad = O.correct_loss(g.xi, lo.get(), q0.get(), 1)
+    assert rel_l2(Qc.get(), oq) < 1e-12
+
+
+def test_korobov_steps_graph_equals_eager():
+    say("korobov steps")
+    vm = 4.8
+    g = O.Grid((16, 16, 16), (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    c.korobov_build(1 << 16, seed=3, s0=0.25)
+    rng = np.random.default_rng(5)
+    f0 = mixture(g.xi, rng)[None]
+    fa = G.DeviceArray.from_host(f0)
+    fb = G.DeviceArray.from_host(f0)
+    c.run_steps(G.METHOD_KOROBOV, fa, 1, 0.02, 5, G.FLAG_CORRECT)
+    c.run_steps(G.METHOD_KOROBOV, fb, 1, 0.02, 5, G.FLAG_CORRECT | G.FLAG_GRAPH)
+    c.sync()
+    A, B = fa.get(), fb.get()
+    assert np.array_equal(A, B)
+    m0 = moments(g.xi, f0, g.d3v)
+    m1 = moments(g.xi, A, g.d3v)
+    assert np.max(np.abs(m1 - m0) / np.abs(m0).max(axis=0)) < 1e-13
+
+
+def test_korobov_fused_equals_two_phase():
+    """The fused LDS kernel and the node-value + reduce pair perform the same
+    operations per entry in the same order: identical Q and loss bits."""
+    say("korobov fused vs two-phase")
+    vm = 4.8
+    for n, M in (((20, 20, 20), 1 << 18), ((32, 32, 32), 1 << 19)):
+        g = O.Grid(n, (-vm,) * 3, (vm,) * 3, vm)
+        c = ctx_for(g)
+        c.korobov_build(M, seed=21, s0=0.25)
+        rng = np.random.default_rng(8)
+        cells = 2
+        f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
+        fd = G.DeviceArray.from_host(f)
+        nu1, g1, q1 = c.evaluate(G.METHOD_KOROBOV, fd, cells, 0)
+        nu2, g2, q2 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_TWO_PHASE)
+        assert np.array_equal(q1.get(), q2.get())
+        assert np.array_equal(nu1.get(), nu2.get())
+        st = O.korobov_stencil(g, M, c.korobov_info()["a"], 0.25)
+        ol, oq = O.korobov_collide(g, st, f)
+        assert rel_l2(q1.get(), oq) < 1e-12
+        # corrected: both paths conserve and agree to FP64 rounding
+        _, _, c1 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_CORRECT)
+        _, _, c2 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_CORRECT | G.FLAG_TWO_PHASE)
+        assert rel_l2(c1.get(), c2.get()) < 1e-12
+        oqc, _, _ = O.correct_loss(g.xi, ol, oq, cells)
+        assert rel_l2(c1.get(), oqc) < 1e-12
+
+
+def test_korobov_relaxation_cfg2_short():
+    """32^3 / sphere cut, 2^21 nodes (the relaxation configuration): 40 corrected steps of dt = 0.02 keep
+    mass, momentum and energy to 1e-13 and f positive; entropy does not rise;
+    graph replay equals eager launches bit for bit."""
+    say("korobov relaxation 32^3")
+    vm = 4.8
+    g = O.Grid((32, 32, 32), (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    c.korobov_build(1 << 21, seed=2026, s0=0.25)
+    rng = np.random.default_rng(2026)
+    f0 = mixture(g.xi, rng)[None]
+    fa = G.DeviceArray.from_host(f0)
+    fb = G.DeviceArray.from_host(f0)
+    H = []
+    prev = f0[0]
+    for blk in range(4):
+        c.run_steps(G.METHOD_KOROBOV, fa, 1, 0.02, 10, G.FLAG_CORRECT | G.FLAG_GRAPH)
+        cur = fa.get()[0]
+        assert cur.min() > 0
+        H.append(float(np.sum(cur * np.log(cur)) * g.d3v))
+        prev = cur
+    c.run_steps(G.METHOD_KOROBOV, fb, 1, 0.02, 40, G.FLAG_CORRECT)
+    assert np.array_equal(fb.get()[0], prev)
+    m0 = moments(g.xi, f0[0], g.d3v)
+    m1 = moments(g.xi, prev, g.d3v)
+    assert np.max(np.abs(m1 - m0) / np.abs(m0).max()) < 1e-13
+    H0 = float(np.sum(f0 * np.log(f0)) * g.d3v)
+    assert all(b <= a + 1e-12 for a, b in zip([H0] + H[:-1], H))
