@@ -129,19 +129,17 @@ def main():
     n = ctx.n_pnt
     nn, ne = info["nodes"], info["entries"]
     # algorithmic bytes per launch (cells = 1)
-    nparts = 256  # fused-kernel blocks for one cell
+    nparts = 256  # blocks per cell of the gather / fused kernels
     alg = {
-        # fused path: 20-byte entry records in CSR order + row pointers + f
-        # (once) + node centres in; Q, loss and moment partials out
-        "kor_fused": ne * 20 + 4 * (n + 1) + 8 * n + 24 * n + 16 * n + nparts * 160,
-        # partials + centres + f + loss + Q in; Q and f out
+        # node records (uint4 + D2) + f in; (D, 2wP) per node out
+        "kor_node": nn * 20 + 8 * n + nn * 16,
+        # CSR entries (node|role + coefficient) + node values (each read at
+        # least once) + row pointers + centres in; Q, loss, moment sums out
+        "kor_gather": ne * 12 + nn * 16 + 4 * (n + 1) + 24 * n + 16 * n + nparts * 160,
+        # moment sums + centres + f + loss + Q in; Q and f out
         "kor_apply": nparts * 160 + 24 * n + 24 * n + 16 * n,
-        # two-phase path: stencil record (6 int32 + r + w) + node values out + f in
-        "kor_values": nn * (24 + 8 + 8) + nn * 32 + 8 * n,
-        # CSR entries + node values (each read once) + row pointers + loss, Q out
-        "kor_reduce": ne * 4 + nn * 32 + 4 * (n + 1) + 16 * n,
-        # single-block correction: f, loss, Q, xi in; Q and f out
-        "correct": 8 * n * 3 + 24 * n + 16 * n,
+        # alternative fused kernel: 20-byte entry records + f + ptr + centres
+        "kor_fused": ne * 20 + 4 * (n + 1) + 8 * n + 24 * n + 16 * n + nparts * 160,
     }
     roof = None
     kstats = {}

@@ -24,7 +24,7 @@ FLAG_CORRECT = 1
 FLAG_GRAPH = 2
 FLAG_TIME = 4
 FLAG_COURANT = 8
-FLAG_TWO_PHASE = 16
+FLAG_FUSED = 16
 
 # every symbol that include/gdt.h declares
 ABI_SYMBOLS = (
@@ -33,7 +33,8 @@ ABI_SYMBOLS = (
     "gdt_grid_info", "gdt_grid_nodes", "gdt_basis", "gdt_build_tables", "gdt_tables", "gdt_set_tables",
     "gdt_korobov_build", "gdt_korobov_info", "gdt_korobov_export", "gdt_korobov_export_csr", "gdt_korobov_pick",
     "gdt_collide_matrix", "gdt_collide_korobov", "gdt_correct", "gdt_evaluate", "gdt_relax", "gdt_run_steps",
-    "gdt_kernel_stats", "gdt_kernel_stats_reset", "gdt_transpose",
+    "gdt_kernel_stats", "gdt_kernel_stats_reset", "gdt_transpose", "gdt_device_synchronize",
+    "gdt_convert_f32_to_f64", "gdt_convert_f64_to_f32",
 )
 
 _lib = None
@@ -86,7 +87,10 @@ def lib():
         "gdt_collide_korobov": [vp, vp, vp, vp, C.c_int, C.c_int],
         "gdt_correct": [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int],
         "gdt_evaluate": [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_uint],
-        "gdt_relax": [vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, C.c_int, C.c_uint],
+        "gdt_relax": [vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, C.c_uint],
+        "gdt_device_synchronize": [],
+        "gdt_convert_f32_to_f64": [vp, vp, C.c_size_t],
+        "gdt_convert_f64_to_f32": [vp, vp, C.c_size_t],
         "gdt_run_steps": [vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, C.c_uint],
         "gdt_kernel_stats": [vp, C.c_char_p, dp, C.POINTER(C.c_long)],
         "gdt_kernel_stats_reset": [vp],
@@ -263,9 +267,10 @@ class Context:
         check(lib().gdt_correct(self.h, int(method), f.ptr, nu.ptr, gain.ptr, Q.ptr, X.ptr if X else None,
                                 int(cells), LAYOUT_CELL_MAJOR))
 
-    def relax(self, f, nu, Q, cells, dt=0.0, courant=0.5, time=None, flags=0, layout=LAYOUT_CELL_MAJOR):
-        check(lib().gdt_relax(self.h, f.ptr, nu.ptr if nu else None, Q.ptr, int(cells), float(dt), float(courant),
-                              time.ptr if time else None, int(layout), int(flags)))
+    def relax(self, f, nu, gain, cells, dt=0.0, courant=0.5, time=None, flags=0):
+        """relax_f: f <- f (1 - nu dt) + gain dt (dt from courant with FLAG_COURANT)."""
+        check(lib().gdt_relax(self.h, f.ptr, nu.ptr, gain.ptr, int(cells), float(dt), float(courant),
+                              time.ptr if time else None, int(flags)))
 
     def run_steps(self, method, f, cells, dt, steps, flags=FLAG_CORRECT):
         check(lib().gdt_run_steps(self.h, int(method), f.ptr, int(cells), float(dt), int(steps), int(flags)))

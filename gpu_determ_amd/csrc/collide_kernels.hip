@@ -3,18 +3,12 @@
 // Layouts: velocity arrays are [cell][node]; "box" arrays cover the whole
 // n_u x n_v x n_w box with zeros outside the sphere cut, "cmp" arrays hold the
 // kept nodes only.  All arithmetic is FP64.
+#include <algorithm>
 #include <cstdio>
+#include <vector>
 
 #include "gdt_internal.h"
-
-#define WAVE 64
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
-}
+#include "kernel_common.h"
 
 // ---------------------------------------------------------------------------
 // layout helpers
@@ -230,102 +224,6 @@ int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells
 }
 
 // ---------------------------------------------------------------------------
-// Korobov projection method
-// ---------------------------------------------------------------------------
-// Node values: per stencil node t and cell c,
-//   P = f_a f_b,  R = f_l f_m (f_ls f_ms / f_l f_m)^r,  D = w (P - R)
-//   vals = (D, (1-r) D, r D, w P)
-// (linear interpolation of the two pair products when one is not positive).
-__global__ __launch_bounds__(256) void k_kor_values(long nn, int npts, const int32_t *__restrict__ idx6,
-                                                    const double *__restrict__ rr, const double *__restrict__ ww,
-                                                    const double *__restrict__ f, double *__restrict__ vals)
-{
-#pragma clang fp contract(off)
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nn) return;
-    const int c = blockIdx.y;
-    const double *fc = f + (size_t)c * npts;
-    const int2 *ip = reinterpret_cast<const int2 *>(idx6 + 6 * t);
-    const int2 i01 = ip[0], i23 = ip[1], i45 = ip[2];
-    const double r = rr[t], w = ww[t];
-    const double P = fc[i01.x] * fc[i01.y];
-    const double Pl = fc[i23.x] * fc[i23.y];
-    double R;
-    if (r == 0.0) {
-        R = Pl;
-    } else {
-        const double Ps = fc[i45.x] * fc[i45.y];
-        if (Pl > 0.0 && Ps > 0.0)
-            R = Pl * exp(r * log(Ps / Pl));
-        else
-            R = (1.0 - r) * Pl + r * Ps;
-    }
-    const double D = w * (P - R);
-    double4 o;
-    o.x = D;
-    o.y = (1.0 - r) * D;
-    o.z = r * D;
-    o.w = w * P;
-    reinterpret_cast<double4 *>(vals)[(size_t)c * nn + t] = o;
-}
-
-int launch_kor_values(gdt_context *h, const double *f, double *vals, int cells)
-{
-    KernelTimer kt(h, "kor_values");
-    hipLaunchKernelGGL(k_kor_values, dim3((unsigned)((h->nn + 255) / 256), cells), dim3(256), 0, h->stream, h->nn,
-                       h->n_pnt, h->kidx.as<int32_t>(), h->kr.as<double>(), h->kw.as<double>(), f, vals);
-    GDT_HIP(hipGetLastError());
-    return GDT_OK;
-}
-
-// Destination-sorted segmented reduction: one wave per destination node;
-// the CSR entry (node << 2 | role) selects -D and +wP (role 0: alpha/beta),
-// (1-r) D (role 1: lambda/mu) or r D (role 2: lambda+s/mu-s).  Lanes stride
-// the segment in order and a fixed butterfly finishes, so the sum order is
-// the same on every run.
-__global__ __launch_bounds__(256) void k_kor_reduce(int npts, long nn, const int32_t *__restrict__ ptr,
-                                                    const uint32_t *__restrict__ ent,
-                                                    const double *__restrict__ vals, double *__restrict__ loss,
-                                                    double *__restrict__ Q)
-{
-#pragma clang fp contract(off)
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int gm = blockIdx.x * (blockDim.x / WAVE) + wave;
-    if (gm >= npts) return;
-    const int c = blockIdx.y;
-    const double *vc = vals + (size_t)c * nn * 4;
-    double sq = 0.0, sl = 0.0;
-    const int e1 = ptr[gm + 1];
-    for (int e = ptr[gm] + lane; e < e1; e += WAVE) {
-        const uint32_t x = ent[e];
-        const size_t t = x >> 2;
-        const uint32_t role = x & 3u;
-        if (role == 0) {
-            const double2 a = *reinterpret_cast<const double2 *>(vc + 4 * t);
-            sq -= a.x;
-            sl += vc[4 * t + 3];
-        } else {
-            sq += vc[4 * t + role];
-        }
-    }
-    sq = wave_sum(sq);
-    sl = wave_sum(sl);
-    if (lane == 0) {
-        Q[(size_t)c * npts + gm] = sq;
-        if (loss) loss[(size_t)c * npts + gm] = sl;
-    }
-}
-
-int launch_kor_reduce(gdt_context *h, const double *vals, double *loss, double *Q, int cells)
-{
-    KernelTimer kt(h, "kor_reduce");
-    hipLaunchKernelGGL(k_kor_reduce, dim3((h->n_pnt + 3) / 4, cells), dim3(256), 0, h->stream, h->n_pnt, h->nn,
-                       h->kptr.as<int32_t>(), h->kent.as<uint32_t>(), vals, loss, Q);
-    GDT_HIP(hipGetLastError());
-    return GDT_OK;
-}
-
-// ---------------------------------------------------------------------------
 // conservation correction (+ optional explicit step), one block per cell
 // ---------------------------------------------------------------------------
 // With psi = (1, xi_x, xi_y, xi_z, |xi|^2) and L = f nu (MATRIX) or the loss
@@ -337,32 +235,6 @@ int launch_kor_reduce(gdt_context *h, const double *vals, double *loss, double *
 // (before_correction, linear_solver, correct_frequency,
 // collision_integral.cu:217-526, solved for every cell).  With dt != 0 the
 // block also applies f <- f + dt Q (evolve_f, :809-833).
-__device__ bool solve5(double M[5][6], double X[5])
-{
-    for (int col = 0; col < 5; col++) {
-        int piv = col;
-        for (int r = col + 1; r < 5; r++)
-            if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
-        if (M[piv][col] == 0.0) return false;
-        if (piv != col)
-            for (int cc = 0; cc < 6; cc++) {
-                const double t = M[col][cc];
-                M[col][cc] = M[piv][cc];
-                M[piv][cc] = t;
-            }
-        for (int r = col + 1; r < 5; r++) {
-            const double m = M[r][col] / M[col][col];
-            for (int cc = col; cc < 6; cc++) M[r][cc] -= m * M[col][cc];
-        }
-    }
-    for (int r = 4; r >= 0; r--) {
-        double s = M[r][5];
-        for (int cc = r + 1; cc < 5; cc++) s -= M[r][cc] * X[cc];
-        X[r] = s / M[r][r];
-    }
-    return true;
-}
-
 // MODE 0: matrix, corrected; 1: Korobov, corrected; 2: matrix, X = 0;
 // 3: Korobov, X = 0.
 template <int MODE>
@@ -402,31 +274,15 @@ __global__ __launch_bounds__(256) void k_correct(int npts, const double *__restr
                 for (int b = a; b < 5; b++) m[q++] += L * psi[a] * psi[b];
         }
         const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+        wave_sum_n<20>(m);
+        if (lane == 0)
 #pragma unroll
-        for (int q = 0; q < 20; q++) {
-            const double s = wave_sum(m[q]);
-            if (lane == 0) part[wave][q] = s;
-        }
+            for (int q = 0; q < 20; q++) part[wave][q] = m[q];
         __syncthreads();
         if (threadIdx.x == 0) {
-            const int nwv = blockDim.x / WAVE;
-            double s[20];
-            for (int q = 0; q < 20; q++) {
-                s[q] = 0.0;
-                for (int w = 0; w < nwv; w++) s[q] += part[w][q];
-            }
-            double M[5][6];
-            int q = 5;
-            for (int a = 0; a < 5; a++)
-                for (int b = a; b < 5; b++) {
-                    M[a][b] = s[q];
-                    M[b][a] = s[q];
-                    q++;
-                }
-            for (int a = 0; a < 5; a++) M[a][5] = s[a];
-            double X[5] = {0, 0, 0, 0, 0};
-            if (!solve5(M, X))
-                for (int a = 0; a < 5; a++) X[a] = 0.0;
+            double sm[20], X[5];
+            for (int q = 0; q < 20; q++) sm[q] = part[0][q] + part[1][q] + part[2][q] + part[3][q];
+            moments_solve(sm, X);
             for (int a = 0; a < 5; a++) Xs[a] = X[a];
         }
         __syncthreads();
@@ -537,20 +393,44 @@ int launch_courant_dt(gdt_context *h, const double *nu, long count, double coura
     return GDT_OK;
 }
 
-__global__ void k_euler(double *__restrict__ f, const double *__restrict__ Q, long count, double dt,
-                        const double *__restrict__ dt_dev, double *__restrict__ time)
+// f <- f (1 - nu dt) + gain dt (evolve_f, collision_integral.cu:809-833),
+// dt from the device (courant) or the argument; time += dt.
+__global__ void k_euler(double *__restrict__ f, const double *__restrict__ nu, const double *__restrict__ gain,
+                        long count, double dt, const double *__restrict__ dt_dev, double *__restrict__ time)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const double d = dt_dev ? dt_dev[0] : dt;
-    if (t < count) f[t] = f[t] + d * Q[t];
+    if (t < count) f[t] = f[t] * (1.0 - nu[t] * d) + gain[t] * d;
     if (t == 0 && time) time[0] += d;
 }
 
-int launch_euler(gdt_context *h, double *f, const double *Q, long count, double dt, const double *dt_dev, double *time)
+int launch_euler(gdt_context *h, double *f, const double *nu, const double *gain, long count, double dt,
+                 const double *dt_dev, double *time)
 {
-    hipLaunchKernelGGL(k_euler, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->stream, f, Q, count, dt, dt_dev,
-                       time);
+    hipLaunchKernelGGL(k_euler, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->stream, f, nu, gain, count, dt,
+                       dt_dev, time);
     GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+__global__ void k_convert(const float *__restrict__ s32, double *__restrict__ d64, long count,
+                          const double *__restrict__ s64, float *__restrict__ d32)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    if (s32)
+        d64[t] = (double)s32[t];
+    else
+        d32[t] = (float)s64[t];
+}
+
+int launch_convert(const float *s32, double *d64, size_t count, const double *s64, float *d32)
+{
+    if (count == 0) return GDT_OK;
+    hipLaunchKernelGGL(k_convert, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, 0, s32, d64, (long)count, s64,
+                       d32);
+    GDT_HIP(hipGetLastError());
+    GDT_HIP(hipDeviceSynchronize());
     return GDT_OK;
 }
 
@@ -621,209 +501,3 @@ int launch_fill(gdt_context *h, double *dst, long count, double value)
     return GDT_OK;
 }
 
-// ---------------------------------------------------------------------------
-// fused projection-method collide (f in LDS, entry records streamed)
-// ---------------------------------------------------------------------------
-// The stencil is stored a second time in CSR order, one 20-byte record per
-// (destination, node, role) entry: the six node indices as uint16 pairs, the
-// integers E0 - E_lm and E_s - E_lm that fix r = num/den, and D2 | role << 16
-// that fixes w = K sqrt(D2).  A block stages f of its cell in LDS; each wave
-// owns destinations and streams their records (coalesced), recomputing the
-// node's Delta per entry, so nothing is gathered from HBM at random.  The
-// per-entry arithmetic is the one of k_kor_values / k_kor_reduce operation for
-// operation (contraction off), so both paths give the same bits.  Each block
-// also writes the 20 moment sums of its destinations for the correction.
-#define FUSED_THREADS 1024
-
-__global__ __launch_bounds__(FUSED_THREADS) void k_kor_fused(int npts, int dpb, const int32_t *__restrict__ ptr,
-                                                              const uint4 *__restrict__ e4,
-                                                              const uint32_t *__restrict__ e1, double K,
-                                                              const double *__restrict__ f,
-                                                              const double *__restrict__ xi,
-                                                              double *__restrict__ loss, double *__restrict__ Q,
-                                                              double *__restrict__ mpart)
-{
-#pragma clang fp contract(off)
-    extern __shared__ double fs[];
-    __shared__ double wpart[FUSED_THREADS / WAVE][20];
-    const int c = blockIdx.y;
-    const double *fc = f + (size_t)c * npts;
-    for (int t = threadIdx.x; t < npts; t += blockDim.x) fs[t] = fc[t];
-    __syncthreads();
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nwv = blockDim.x / WAVE;
-    const int g0 = blockIdx.x * dpb;
-    const int g1 = min(npts, g0 + dpb);
-    double m[20];
-#pragma unroll
-    for (int q = 0; q < 20; q++) m[q] = 0.0;
-    for (int gm = g0 + wave; gm < g1; gm += nwv) {
-        double sq = 0.0, sl = 0.0;
-        const int eend = ptr[gm + 1];
-        for (int e = ptr[gm] + lane; e < eend; e += WAVE) {
-            const uint4 x = e4[e];
-            const uint32_t y = e1[e];
-            const int num = (int)(short)(x.w & 0xFFFFu), den = (int)(short)(x.w >> 16);
-            const double P = fs[x.x & 0xFFFFu] * fs[x.x >> 16];
-            const double Pl = fs[x.y & 0xFFFFu] * fs[x.y >> 16];
-            double r = 0.0, R;
-            if (num == 0) {
-                R = Pl;
-            } else {
-                r = (double)num / (double)den;
-                const double Ps = fs[x.z & 0xFFFFu] * fs[x.z >> 16];
-                if (Pl > 0.0 && Ps > 0.0)
-                    R = Pl * exp(r * log(Ps / Pl));
-                else
-                    R = (1.0 - r) * Pl + r * Ps;
-            }
-            const double w = K * sqrt((double)(y & 0xFFFFu));
-            const double D = w * (P - R);
-            const uint32_t role = y >> 16;
-            if (role == 0) {
-                sq -= D;
-                const double wp = w * P;
-                sl += wp;
-            } else if (role == 1) {
-                const double t1 = (1.0 - r) * D;
-                sq += t1;
-            } else {
-                const double t2 = r * D;
-                sq += t2;
-            }
-        }
-        sq = wave_sum(sq);
-        sl = wave_sum(sl);
-        if (lane == 0) {
-            Q[(size_t)c * npts + gm] = sq;
-            if (loss) loss[(size_t)c * npts + gm] = sl;
-            const double x0 = xi[3 * gm], y0 = xi[3 * gm + 1], z0 = xi[3 * gm + 2];
-            const double psi[5] = {1.0, x0, y0, z0, x0 * x0 + y0 * y0 + z0 * z0};
-#pragma unroll
-            for (int a = 0; a < 5; a++) m[a] += sq * psi[a];
-            int q = 5;
-#pragma unroll
-            for (int a = 0; a < 5; a++)
-#pragma unroll
-                for (int b = a; b < 5; b++) m[q++] += sl * psi[a] * psi[b];
-        }
-    }
-    if (lane == 0)
-#pragma unroll
-        for (int q = 0; q < 20; q++) wpart[wave][q] = m[q];
-    __syncthreads();
-    if (threadIdx.x < 20) {
-        double s = 0.0;
-        for (int w = 0; w < nwv; w++) s += wpart[w][threadIdx.x];
-        mpart[((size_t)c * gridDim.x + blockIdx.x) * 20 + threadIdx.x] = s;
-    }
-}
-
-int kor_fused_blocks(gdt_context *h, int cells)
-{
-    (void)h;
-    const int target = 256;  // one block per CU at one block of LDS per CU
-    int b = (target + cells - 1) / cells;
-    return b < 1 ? 1 : b;
-}
-
-// Called once when the entry records are built (not during graph capture).
-int kor_fused_setup(gdt_context *h)
-{
-    const int lds = (int)(sizeof(double) * h->n_pnt);
-    GDT_HIP(hipFuncSetAttribute((const void *)k_kor_fused, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-    return GDT_OK;
-}
-
-int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells)
-{
-    const int B = kor_fused_blocks(h, cells);
-    const int dpb = (h->n_pnt + B - 1) / B;
-    const size_t lds = sizeof(double) * h->n_pnt;
-    KernelTimer kt(h, "kor_fused");
-    hipLaunchKernelGGL(k_kor_fused, dim3(B, cells), dim3(FUSED_THREADS), lds, h->stream, h->n_pnt, dpb,
-                       h->kptr.as<int32_t>(), h->ke4.as<uint4>(), h->ke1.as<uint32_t>(), h->kK, f,
-                       h->xi.as<double>(), loss, Q, mpart);
-    GDT_HIP(hipGetLastError());
-    return GDT_OK;
-}
-
-// Correction from the fused kernel's moment partials, then (dt != 0) the
-// explicit step.  Every block reduces the partials of its cell in the same
-// order and solves the same 5x5 system, then handles 256 nodes.
-template <bool CORR>
-__global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, const double *__restrict__ mpart,
-                                                   const double *__restrict__ xi, const double *f,
-                                                   const double *__restrict__ loss, const double *__restrict__ Qin,
-                                                   double *__restrict__ Qout, double *__restrict__ nu_out,
-                                                   double *__restrict__ Xout, double dt, double *f_upd)
-{
-    __shared__ double part[4][20];
-    __shared__ double Xs[5];
-    const int c = blockIdx.y;
-    if (CORR) {
-        double m[20];
-#pragma unroll
-        for (int q = 0; q < 20; q++) m[q] = 0.0;
-        for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-            const double *mp = mpart + ((size_t)c * nparts + p) * 20;
-#pragma unroll
-            for (int q = 0; q < 20; q++) m[q] += mp[q];
-        }
-        const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-#pragma unroll
-        for (int q = 0; q < 20; q++) {
-            const double s = wave_sum(m[q]);
-            if (lane == 0) part[wave][q] = s;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s[20];
-            for (int q = 0; q < 20; q++) s[q] = part[0][q] + part[1][q] + part[2][q] + part[3][q];
-            double M[5][6];
-            int q = 5;
-            for (int a = 0; a < 5; a++)
-                for (int b = a; b < 5; b++) {
-                    M[a][b] = s[q];
-                    M[b][a] = s[q];
-                    q++;
-                }
-            for (int a = 0; a < 5; a++) M[a][5] = s[a];
-            double X[5] = {0, 0, 0, 0, 0};
-            if (!solve5(M, X))
-                for (int a = 0; a < 5; a++) X[a] = 0.0;
-            for (int a = 0; a < 5; a++) Xs[a] = X[a];
-        }
-    } else if (threadIdx.x < 5) {
-        Xs[threadIdx.x] = 0.0;
-    }
-    __syncthreads();
-    if (Xout && blockIdx.x == 0 && threadIdx.x < 5) Xout[5 * c + threadIdx.x] = Xs[threadIdx.x];
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= npts) return;
-    const size_t o = (size_t)c * npts + v;
-    const double x = xi[3 * v], y = xi[3 * v + 1], z = xi[3 * v + 2];
-    const double e = x * x + y * y + z * z;
-    const double s = Xs[0] + Xs[1] * x + Xs[2] * y + Xs[3] * z + Xs[4] * e;
-    const double L = loss[o];
-    const double fv = f[o];
-    const double q = Qin[o] - L * s;
-    Qout[o] = q;
-    if (nu_out) nu_out[o] = fv > 0.0 ? (L / fv) * (1.0 + s) : 0.0;
-    if (dt != 0.0) f_upd[o] = fv + dt * q;
-}
-
-int launch_kor_apply(gdt_context *h, bool correct, const double *mpart, const double *f, const double *loss,
-                     const double *Qin, double *Qout, double *nu_out, double *X, int cells, double dt, double *f_upd)
-{
-    const int nparts = kor_fused_blocks(h, cells);
-    const dim3 grid((h->n_pnt + 255) / 256, cells);
-    KernelTimer kt(h, "kor_apply");
-    if (correct)
-        hipLaunchKernelGGL(k_kor_apply<true>, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, mpart,
-                           h->xi.as<double>(), f, loss, Qin, Qout, nu_out, X, dt, f_upd);
-    else
-        hipLaunchKernelGGL(k_kor_apply<false>, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, mpart,
-                           h->xi.as<double>(), f, loss, Qin, Qout, nu_out, X, dt, f_upd);
-    GDT_HIP(hipGetLastError());
-    return GDT_OK;
-}

@@ -371,7 +371,9 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
     GDT_CHECK(a == 0 || (a & 1u), GDT_E_ARG, "gdt_korobov_build: generator must be odd");
     if (a == 0) a = host_korobov_pick(M, seed, 8, nullptr);
     const double d = h->d[0];
-    const double K = 2.0 * 3.141592653589793 * s0 * (double)h->n_box * (double)h->n_box * d * d * d * d / (double)M;
+    // node weight K sqrt(D2): pi s0 n_box^2 d^4 / M (weak form of the symmetric
+    // direct-minus-inverse node term; the loss carries twice this weight)
+    const double K = 3.141592653589793 * s0 * (double)h->n_box * (double)h->n_box * d * d * d * d / (double)M;
 
     DevBuf t_idx, t_r, t_w, t_v, t_i;
     GDT_CHECK(t_idx.ensure(sizeof(int32_t) * 6 * (size_t)M) == 0 && t_r.ensure(sizeof(double) * M) == 0 &&
@@ -440,28 +442,82 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
     GDT_HIP(hipMemcpy(h->kptr.p, ptr.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice));
     GDT_HIP(hipMemcpy(h->kent.p, ent.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
 
-    // entry records for the fused kernel (needs 16-bit node indices and an
-    // LDS-resident f)
-    bool stream_ok = np <= 65535 && (size_t)np * sizeof(double) <= 150 * 1024;
+    h->nn = nn;  // read by the wave partition below
+    h->nent = ne;
+    // packed records (16-bit node indices, the f of a cell in LDS)
+    bool stream_ok = np <= 65535 && (size_t)np * sizeof(double) <= 150 * 1024 && np < (1 << 18);
     for (long t = 0; t < nn && stream_ok; t++)
-        stream_ok = std::abs(cint[3 * t]) < 32768 && std::abs(cint[3 * t + 1]) < 32768 && cint[3 * t + 2] < 65536;
+        stream_ok = std::abs(cint[3 * t]) < 32768 && std::abs(cint[3 * t + 1]) < 32768 && cint[3 * t + 2] < 4096;
+    GDT_CHECK(stream_ok, GDT_E_ARG,
+              "gdt_korobov_build: grid of %d nodes is beyond the packed-record limits (<= 19,200 nodes)", np);
     h->have_stream = false;
-    if (stream_ok) {
+    h->wb_blocks = 0;
+    h->h_kptr = ptr;
+    {
+        auto pack = [&](long t, uint32_t *o4) {
+            const int32_t *o = &cidx6[6 * (size_t)t];
+            o4[0] = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+            o4[1] = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+            o4[2] = (uint32_t)o[4] | ((uint32_t)o[5] << 16);
+            o4[3] = ((uint32_t)cint[3 * t] & 0xFFFFu) | (((uint32_t)cint[3 * t + 1] & 0xFFFFu) << 16);
+        };
+        std::vector<uint32_t> n4(4 * (size_t)nn), n1(nn);
+        for (long t = 0; t < nn; t++) {
+            pack(t, &n4[4 * t]);
+            n1[t] = (uint32_t)cint[3 * t + 2];
+        }
+        std::vector<int32_t> dest_of(ne);
+        for (int v = 0; v < np; v++)
+            for (int e = ptr[v]; e < ptr[v + 1]; e++) dest_of[e] = v;
         std::vector<uint32_t> e4(4 * (size_t)ne), e1(ne);
+        std::vector<double> coef(ne);
         for (long e = 0; e < ne; e++) {
             const uint32_t t = ent[e] >> 2, role = ent[e] & 3u;
-            const int32_t *o = &cidx6[6 * (size_t)t];
-            e4[4 * e + 0] = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-            e4[4 * e + 1] = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-            e4[4 * e + 2] = (uint32_t)o[4] | ((uint32_t)o[5] << 16);
-            e4[4 * e + 3] = ((uint32_t)cint[3 * t] & 0xFFFFu) | (((uint32_t)cint[3 * t + 1] & 0xFFFFu) << 16);
-            e1[e] = (uint32_t)cint[3 * t + 2] | (role << 16);
+            pack(t, &e4[4 * e]);
+            e1[e] = (uint32_t)cint[3 * t + 2] | (role << 12) | ((uint32_t)dest_of[e] << 14);
+            coef[e] = role == 0 ? -1.0 : (role == 1 ? 1.0 - cr[t] : cr[t]);
         }
-        GDT_CHECK(h->ke4.ensure(sizeof(uint32_t) * 4 * ne) == 0 && h->ke1.ensure(sizeof(uint32_t) * ne) == 0,
-                  GDT_E_NOMEM, "gdt_korobov_build: entry-record allocation failed");
+        GDT_CHECK(h->kn4.ensure(sizeof(uint32_t) * 4 * nn) == 0 && h->kn1.ensure(sizeof(uint32_t) * nn) == 0 &&
+                      h->kcoef.ensure(sizeof(double) * ne) == 0 && h->ke4.ensure(sizeof(uint32_t) * 4 * ne) == 0 &&
+                      h->ke1.ensure(sizeof(uint32_t) * ne) == 0,
+                  GDT_E_NOMEM, "gdt_korobov_build: record allocation failed");
+        GDT_HIP(hipMemcpy(h->kn4.p, n4.data(), sizeof(uint32_t) * 4 * nn, hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kn1.p, n1.data(), sizeof(uint32_t) * nn, hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kcoef.p, coef.data(), sizeof(double) * ne, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke4.p, e4.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke1.p, e1.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
-        int rc2 = kor_fused_setup(h);
+        // per node-group CSRs of the gather: group of node t is
+        // t * KOR_NGROUPS / nn (contiguous node ranges); within a group the
+        // entries are sorted by destination, then by (node, slot)
+        const int G = KOR_NGROUPS;
+        std::vector<int32_t> gp((size_t)G * (np + 1), 0);
+        auto grp = [&](uint32_t t) { return (int)(((uint64_t)t * G) / (uint64_t)nn); };
+        for (long e = 0; e < ne; e++) gp[(size_t)grp(ent[e] >> 2) * (np + 1) + dest_of[e] + 1]++;
+        long run = 0;
+        for (int g = 0; g < G; g++) {
+            int32_t *pg = &gp[(size_t)g * (np + 1)];
+            pg[0] = (int32_t)run;
+            for (int v = 0; v < np; v++) {
+                run += pg[v + 1];
+                pg[v + 1] = (int32_t)run;
+            }
+        }
+        std::vector<uint32_t> gent(ne);
+        std::vector<double> gcoef(ne);
+        std::vector<int32_t> fill(gp);
+        for (long e = 0; e < ne; e++) {  // ent is destination-sorted with (node, slot) order inside
+            const int g = grp(ent[e] >> 2);
+            const int32_t pos = fill[(size_t)g * (np + 1) + dest_of[e]]++;
+            gent[pos] = ent[e];
+            gcoef[pos] = coef[e];
+        }
+        GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * gp.size()) == 0 && h->kgent.ensure(sizeof(uint32_t) * ne) == 0 &&
+                      h->kgcoef.ensure(sizeof(double) * ne) == 0,
+                  GDT_E_NOMEM, "gdt_korobov_build: group CSR allocation failed");
+        GDT_HIP(hipMemcpy(h->kgptr.p, gp.data(), sizeof(int32_t) * gp.size(), hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kgent.p, gent.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kgcoef.p, gcoef.data(), sizeof(double) * ne, hipMemcpyHostToDevice));
+        int rc2 = kor_setup(h, 1);
         if (rc2) return rc2;
         h->have_stream = true;
     }
@@ -547,40 +603,56 @@ static int collide_matrix_cm(gdt_context *h, const double *f_cm, double *nu, dou
     return GDT_OK;
 }
 
-static bool use_fused(gdt_context *h, unsigned flags) { return h->have_stream && !(flags & GDT_FLAG_TWO_PHASE); }
+static bool use_fused(unsigned flags) { return (flags & GDT_FLAG_FUSED) != 0; }
 
-static int ensure_mpart(gdt_context *h, int cells)
+// Scratch of the Korobov path for `cells` cells: moment sums, node values,
+// group partials, and the fused path's wave partition (outside any capture).
+static int kor_prepare(gdt_context *h, int cells)
 {
-    const size_t need = sizeof(double) * 20 * (size_t)cells * kor_fused_blocks(h, cells);
-    GDT_CHECK(h->s_mpart.ensure(need) == 0, GDT_E_NOMEM, "moment-partial scratch allocation failed");
-    return GDT_OK;
+    const int nb = std::max(kor_blocks(h, cells), kor_gather_blocks(h, cells));
+    const size_t mp = sizeof(double) * 20 * (size_t)cells * nb;
+    const size_t dw = sizeof(double) * 2 * (size_t)h->nn * cells;
+    const size_t pt = sizeof(double) * 2 * (size_t)KOR_NGROUPS * h->n_pnt * cells;
+    GDT_CHECK(h->s_mpart.ensure(mp) == 0 && h->s_vals.ensure(dw) == 0 && h->s_part.ensure(pt) == 0, GDT_E_NOMEM,
+              "Korobov scratch allocation failed (%zu bytes of node values)", dw);
+    return kor_setup(h, cells);
 }
 
-// Korobov collide into cell-major loss / Q.  Fused path: one kernel, moment
-// partials left in s_mpart.  Two-phase path: node values then the
-// destination reduction, in chunks of cells that keep the node-value scratch
-// under 2 GiB.
+// Group partials of loss and Q (and the moment sums in s_mpart) of the
+// Korobov operator for `cells` cell-major cells.
+struct KorPartials {
+    int G = 1, nparts = 1;
+    double *L = nullptr, *Q = nullptr;
+};
+
+static int kor_partials(gdt_context *h, const double *f_cm, int cells, unsigned flags, KorPartials &kp)
+{
+    int rc = kor_prepare(h, cells);
+    if (rc) return rc;
+    if (use_fused(flags)) {
+        kp.G = 1;
+        kp.nparts = kor_blocks(h, cells);
+        kp.L = h->s_part.as<double>();
+        kp.Q = kp.L + (size_t)cells * h->n_pnt;
+        return launch_kor_fused(h, f_cm, kp.L, kp.Q, h->s_mpart.as<double>(), cells);
+    }
+    kp.G = KOR_NGROUPS;
+    kp.nparts = kor_gather_blocks(h, cells);
+    kp.L = h->s_part.as<double>();
+    kp.Q = kp.L + (size_t)KOR_NGROUPS * cells * h->n_pnt;
+    if ((rc = launch_kor_node(h, f_cm, h->s_vals.as<double>(), cells))) return rc;
+    return launch_kor_gather(h, h->s_vals.as<double>(), kp.L, kp.Q, h->s_mpart.as<double>(), cells);
+}
+
+// Korobov collide into cell-major loss / Q (uncorrected).
 static int collide_korobov_cm(gdt_context *h, const double *f_cm, double *loss, double *Q, int cells,
                               unsigned flags)
 {
-    if (use_fused(h, flags)) {
-        int rc = ensure_mpart(h, cells);
-        if (rc) return rc;
-        return launch_kor_fused(h, f_cm, loss, Q, h->s_mpart.as<double>(), cells);
-    }
-    const size_t per_cell = sizeof(double) * 4 * (size_t)h->nn;
-    int chunk = (int)std::max<size_t>(1, ((size_t)2 << 30) / per_cell);
-    chunk = std::min(chunk, cells);
-    GDT_CHECK(h->s_vals.ensure(per_cell * chunk) == 0, GDT_E_NOMEM, "node-value scratch allocation failed");
-    for (int c0 = 0; c0 < cells; c0 += chunk) {
-        const int cc = std::min(chunk, cells - c0);
-        const size_t o = (size_t)c0 * h->n_pnt;
-        int rc = launch_kor_values(h, f_cm + o, h->s_vals.as<double>(), cc);
-        if (rc) return rc;
-        rc = launch_kor_reduce(h, h->s_vals.as<double>(), loss ? loss + o : nullptr, Q + o, cc);
-        if (rc) return rc;
-    }
-    return GDT_OK;
+    KorPartials kp;
+    int rc = kor_partials(h, f_cm, cells, flags, kp);
+    if (rc) return rc;
+    return launch_kor_apply(h, false, kp.G, kp.nparts, h->s_mpart.as<double>(), f_cm, kp.L, kp.Q, Q, loss, nullptr,
+                            nullptr, cells, 0.0, nullptr);
 }
 
 int gdt_collide_matrix(gdt_handle h, const double *f, double *nu, double *gain, int cells, int layout)
@@ -659,16 +731,12 @@ int gdt_evaluate(gdt_handle h, int method, const double *f, double *nu, double *
         } else if (method == GDT_METHOD_KOROBOV) {
             GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_evaluate: stencil not built");
             double *L = h->s_loss.as<double>();
-            if ((rc = collide_korobov_cm(h, fcm, L, g_c, cells, flags))) goto done;
-            // g_c holds the raw Q; correct into q_c (nu_c = corrected loss frequency)
-            if (use_fused(h, flags)) {
-                if ((rc = launch_kor_apply(h, (flags & GDT_FLAG_CORRECT) != 0, h->s_mpart.as<double>(), fcm, L, g_c,
-                                           q_c, nu_c, nullptr, cells, 0.0, nullptr)))
-                    goto done;
-            } else {
-                const int m = (flags & GDT_FLAG_CORRECT) ? GDT_METHOD_KOROBOV : -2;
-                if ((rc = launch_correct(h, m, fcm, L, g_c, nu_c, q_c, nullptr, cells, 0.0, nullptr))) goto done;
-            }
+            KorPartials kp;
+            if ((rc = kor_partials(h, fcm, cells, flags, kp))) goto done;
+            // q_c = corrected Q, nu_c = corrected loss frequency, L = loss
+            if ((rc = launch_kor_apply(h, (flags & GDT_FLAG_CORRECT) != 0, kp.G, kp.nparts, h->s_mpart.as<double>(),
+                                       fcm, kp.L, kp.Q, q_c, L, nu_c, nullptr, cells, 0.0, nullptr)))
+                goto done;
             // gain = Q + f nu_c
             if ((rc = launch_correct(h, -3, fcm, nu_c, q_c, nullptr, g_c, nullptr, cells, 0.0, nullptr))) goto done;
         } else {
@@ -691,21 +759,37 @@ done:
     return rc;
 }
 
-int gdt_relax(gdt_handle h, double *f, const double *nu, const double *Q, int cells, double dt, double courant,
-              double *time, int layout, unsigned flags)
+int gdt_relax(gdt_handle h, double *f, const double *nu, const double *gain, int cells, double dt, double courant,
+              double *time, unsigned flags)
 {
     H_CHECK(h);
-    GDT_CHECK(f && Q && cells > 0, GDT_E_ARG, "gdt_relax: bad arguments");
-    GDT_CHECK(layout == GDT_LAYOUT_CELL_MAJOR || layout == GDT_LAYOUT_NODE_MAJOR, GDT_E_ARG, "gdt_relax: layout");
-    const long count = (long)cells * h->n_pnt;  // elementwise: layout-independent
+    GDT_CHECK(f && nu && gain && cells > 0, GDT_E_ARG, "gdt_relax: bad arguments");
+    const long count = (long)cells * h->n_pnt;  // element-wise: layout-independent
     if (flags & GDT_FLAG_COURANT) {
-        GDT_CHECK(nu, GDT_E_ARG, "gdt_relax: courant step needs nu");
         GDT_CHECK(h->s_dt.ensure(sizeof(double)) == 0, GDT_E_NOMEM, "gdt_relax: scratch");
         int rc = launch_courant_dt(h, nu, count, courant, h->s_dt.as<double>());
         if (rc) return rc;
-        return launch_euler(h, f, Q, count, 0.0, h->s_dt.as<double>(), time);
+        return launch_euler(h, f, nu, gain, count, 0.0, h->s_dt.as<double>(), time);
     }
-    return launch_euler(h, f, Q, count, dt, nullptr, time);
+    return launch_euler(h, f, nu, gain, count, dt, nullptr, time);
+}
+
+int gdt_device_synchronize(void)
+{
+    GDT_HIP(hipDeviceSynchronize());
+    return GDT_OK;
+}
+
+int gdt_convert_f32_to_f64(const float *src, double *dst, size_t count)
+{
+    GDT_CHECK(src && dst, GDT_E_ARG, "gdt_convert_f32_to_f64: null pointer");
+    return launch_convert(src, dst, count, nullptr, nullptr);
+}
+
+int gdt_convert_f64_to_f32(const double *src, float *dst, size_t count)
+{
+    GDT_CHECK(src && dst, GDT_E_ARG, "gdt_convert_f64_to_f32: null pointer");
+    return launch_convert(nullptr, nullptr, count, src, dst);
 }
 
 // One explicit step of cell-major f (no host synchronisation).
@@ -714,11 +798,10 @@ static int one_step(gdt_context *h, int method, double *f, int cells, double dt,
     int rc;
     const bool corr = (flags & GDT_FLAG_CORRECT) != 0;
     if (method == GDT_METHOD_KOROBOV) {
-        double *L = h->s_loss.as<double>(), *Q0 = h->s_out2.as<double>(), *Q = h->s_Q.as<double>();
-        if ((rc = collide_korobov_cm(h, f, L, Q0, cells, flags))) return rc;
-        if (use_fused(h, flags))
-            return launch_kor_apply(h, corr, h->s_mpart.as<double>(), f, L, Q0, Q, nullptr, nullptr, cells, dt, f);
-        return launch_correct(h, corr ? GDT_METHOD_KOROBOV : -2, f, L, Q0, nullptr, Q, nullptr, cells, dt, f);
+        KorPartials kp;
+        if ((rc = kor_partials(h, f, cells, flags, kp))) return rc;
+        return launch_kor_apply(h, corr, kp.G, kp.nparts, h->s_mpart.as<double>(), f, kp.L, kp.Q,
+                                h->s_Q.as<double>(), nullptr, nullptr, nullptr, cells, dt, f);
     }
     double *nu = h->s_nu.as<double>(), *g = h->s_out2.as<double>(), *Q = h->s_Q.as<double>();
     if ((rc = collide_matrix_cm(h, f, nu, g, cells))) return rc;
@@ -740,13 +823,9 @@ int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int
         const size_t nbox = (size_t)cells * h->n_box;
         GDT_CHECK(h->s_fbox.ensure(sizeof(double) * nbox) == 0 && h->s_out1.ensure(sizeof(double) * nbox) == 0,
                   GDT_E_NOMEM, "gdt_run_steps: scratch allocation failed");
-    } else if (use_fused(h, flags)) {
-        int rc0 = ensure_mpart(h, cells);
-        if (rc0) return rc0;
     } else {
-        const size_t per_cell = sizeof(double) * 4 * (size_t)h->nn;
-        const int chunk = std::min<int>(cells, (int)std::max<size_t>(1, ((size_t)2 << 30) / per_cell));
-        GDT_CHECK(h->s_vals.ensure(per_cell * chunk) == 0, GDT_E_NOMEM, "gdt_run_steps: scratch allocation failed");
+        int rc0 = kor_prepare(h, cells);
+        if (rc0) return rc0;
     }
     const bool graph = (flags & GDT_FLAG_GRAPH) && !(flags & GDT_FLAG_TIME);
     h->timing = (flags & GDT_FLAG_TIME) != 0;
@@ -767,7 +846,7 @@ int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int
             break;
         }
     if (chunk < 8 && steps > 64) chunk = 64;
-    const unsigned key = flags & (GDT_FLAG_CORRECT | GDT_FLAG_TWO_PHASE);
+    const unsigned key = flags & (GDT_FLAG_CORRECT | GDT_FLAG_FUSED);
     if (!h->graph_exec || h->g_method != method || h->g_cells != cells || h->g_dt != dt || h->g_flags != key ||
         h->g_f != f || h->g_chunk != chunk) {
         if (h->graph_exec) {

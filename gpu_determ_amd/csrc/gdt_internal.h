@@ -67,14 +67,18 @@ struct gdt_context {
     double ks0 = 0.0, kK = 0.0;
     long nn = 0, nent = 0;
     DevBuf kidx, kr, kw, knode, kptr, kent;
-    // fused path: per-entry records in CSR order (uint4 node pairs + energy
-    // numerator/denominator, uint32 D2 | role << 16)
+    // per-node records (uint4 index pairs + num | den, uint32 D2), per-entry
+    // coefficients (-1, 1-r, r) and, for the fused path, per-entry records
+    // (uint4 as the node's, uint32 D2 | role << 12 | destination << 14)
     bool have_stream = false;
-    DevBuf ke4, ke1;
-    int fused_blocks = 0;  // blocks per cell of the fused kernel (0 = not chosen)
+    DevBuf kn4, kn1, kcoef, ke4, ke1, kwb;
+    DevBuf kgptr, kgent, kgcoef;  // per node-group CSRs of the gather
+    std::vector<int32_t> h_kptr;  // host copy of the CSR row pointers
+    int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
+    int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
 
     // scratch
-    DevBuf s_fbox, s_out1, s_out2, s_vals, s_loss, s_Q, s_nu, s_tmp, s_dt, s_mpart;
+    DevBuf s_fbox, s_out1, s_out2, s_vals, s_loss, s_Q, s_nu, s_tmp, s_dt, s_mpart, s_part;
 
     // step graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -108,20 +112,25 @@ int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells);
 int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells);
 int launch_cmp_to_box(gdt_context *h, const double *f_cmp, double *f_box, int cells, int layout);
 int launch_box_to_cmp(gdt_context *h, const double *f_box, double *f_cmp, int cells, int layout);
-int launch_kor_values(gdt_context *h, const double *f, double *vals, int cells);
-int launch_kor_reduce(gdt_context *h, const double *vals, double *loss, double *Q, int cells);
-// fused path: Q, loss and per-block moment partials in one pass; then the
-// correction (+ optional Euler step) from the partials
-int kor_fused_blocks(gdt_context *h, int cells);
-int kor_fused_setup(gdt_context *h);
+// ---- Korobov kernels (korobov_kernels.hip) --------------------------------
+int kor_blocks(gdt_context *h, int cells);
+int kor_setup(gdt_context *h, int cells);
+int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells);
+int kor_gather_blocks(gdt_context *h, int cells);
+int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Qpart, double *mpart, int cells);
+#define KOR_NGROUPS 8
 int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells);
-int launch_kor_apply(gdt_context *h, bool correct, const double *mpart, const double *f, const double *loss,
-                     const double *Qin, double *Qout, double *nu_out, double *X, int cells, double dt, double *f_upd);
+// G group partials [cell][G][node] of Q and loss (G = 1: plain arrays)
+int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const double *mpart, const double *f,
+                     const double *Lpart, const double *Qpart, double *Qout, double *Lout, double *nu_out, double *X,
+                     int cells, double dt, double *f_upd);
 int launch_correct(gdt_context *h, int method, const double *f, const double *nu, const double *gain,
                    double *nu_out, double *Q, double *X, int cells, double dt, double *f_upd);
 int launch_transpose(gdt_context *h, const double *src, double *dst, int rows, int cols);
 int launch_courant_dt(gdt_context *h, const double *nu, long count, double courant, double *dt_out);
-int launch_euler(gdt_context *h, double *f, const double *Q, long count, double dt, const double *dt_dev,
-                 double *time);
+int launch_euler(gdt_context *h, double *f, const double *nu, const double *gain, long count, double dt,
+                 const double *dt_dev, double *time);
+// f32 -> f64 when (s32, d64) given, else f64 -> f32; default stream, synchronous
+int launch_convert(const float *s32, double *d64, size_t count, const double *s64, float *d32);
 int launch_basis(gdt_context *h, double *dst);
 int launch_fill(gdt_context *h, double *dst, long count, double value);

@@ -47,8 +47,9 @@ extern "C" {
 #define GDT_FLAG_GRAPH 2u      /* replay the step sequence as a hipGraph    */
 #define GDT_FLAG_TIME 4u       /* time each kernel with hipEvents           */
 #define GDT_FLAG_COURANT 8u    /* dt = courant / max nu (relax_f semantics) */
-#define GDT_FLAG_TWO_PHASE 16u /* Korobov: node-value + reduce kernels instead
-                                  of the fused LDS kernel                   */
+#define GDT_FLAG_FUSED 16u     /* Korobov: one fused kernel recomputing each
+                                  node's term per CSR entry, instead of the
+                                  node-value + gather pair                  */
 
 typedef struct gdt_context *gdt_handle;
 
@@ -72,6 +73,13 @@ int gdt_memcpy_htod(void *dst, const void *src, size_t bytes);
 int gdt_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int gdt_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int gdt_memset(void *dptr, int value, size_t bytes);
+/* Waits for all work on the current device (cudaThreadSynchronize in the
+ * reference, e.g. collision_integral.cu:600). */
+int gdt_device_synchronize(void);
+/* Element-wise precision conversion of device arrays (the reference's FP32
+ * arrays enter and leave the FP64 core through these). */
+int gdt_convert_f32_to_f64(const float *src, double *dst, size_t count);
+int gdt_convert_f64_to_f32(const double *src, float *dst, size_t count);
 
 /* ---- context ------------------------------------------------------------ */
 /* Creates the grid on `device`: node lists, centres and the moment basis.
@@ -123,7 +131,8 @@ uint32_t gdt_korobov_pick(uint32_t M, uint64_t seed, int ncand, double *p2);
  * all 512 boxes, intended output slot) for `cells` cells.  Either output may
  * be NULL. */
 int gdt_collide_matrix(gdt_handle h, const double *f, double *nu, double *gain, int cells, int layout);
-/* Korobov method: loss = f nu and Q (uncorrected) for `cells` cells. */
+/* Korobov method: loss = f nu and Q (uncorrected) for `cells` cells.
+ * Requires at most 19,200 grid nodes (f of a cell is staged in LDS). */
 int gdt_collide_korobov(gdt_handle h, const double *f, double *loss, double *Q, int cells, int layout);
 /* Conservation correction (before_correction / linear_solver /
  * correct_frequency, collision_integral.cu:217-526), for every cell:
@@ -137,11 +146,12 @@ int gdt_correct(gdt_handle h, int method, const double *f, double *nu, const dou
  * loss frequency; KOROBOV: loss/f), gain (KOROBOV: Q + loss), Q. */
 int gdt_evaluate(gdt_handle h, int method, const double *f, double *nu, double *gain, double *Q, int cells,
                  int layout, unsigned flags);
-/* relax_f (collision_integral.cu:835-880): f <- f + dt Q for all cells, with
- * dt from courant (flags & GDT_FLAG_COURANT: dt = min courant/nu) or `dt`.
- * time (device double, may be NULL) += dt. */
-int gdt_relax(gdt_handle h, double *f, const double *nu, const double *Q, int cells, double dt, double courant,
-              double *time, int layout, unsigned flags);
+/* relax_f (collision_integral.cu:835-880): f <- f (1 - nu dt) + gain dt for
+ * all cells, with dt = min courant / nu over every value (flags &
+ * GDT_FLAG_COURANT, get_delta_t :776-807) or the given dt.  time (device
+ * double, may be NULL) += dt.  Element-wise, so any layout. */
+int gdt_relax(gdt_handle h, double *f, const double *nu, const double *gain, int cells, double dt, double courant,
+              double *time, unsigned flags);
 /* `steps` explicit steps df/dt = Q(f,f) of `cells` cells (cell-major),
  * each: collide, correct (if flags & GDT_FLAG_CORRECT), f += dt Q.  With
  * GDT_FLAG_GRAPH the step is captured once as a hipGraph and replayed. */

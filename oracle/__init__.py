@@ -219,9 +219,9 @@ def sincos_2pi(t):
 
 
 def korobov_K(g, M, s0):
-    """Node-weight scale 2 pi s0 n_box^2 d^4 / M (equal spacing on all axes)."""
+    """Node-weight scale pi s0 n_box^2 d^4 / M (equal spacing on all axes)."""
     d = g.d[0]
-    return 2.0 * np.pi * s0 * float(g.n_box) ** 2 * d ** 4 / float(M)
+    return np.pi * s0 * float(g.n_box) ** 2 * d ** 4 / float(M)
 
 
 def korobov_stencil(g, M, a, s0):

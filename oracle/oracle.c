@@ -574,10 +574,17 @@ void orc_euler(double *f, const double *Q, long count, double dt)
  * Per node and cell: P = f_a f_b,  R = (f_l f_m)^(1-r) (f_ls f_ms)^r,
  * Delta = P - R, and with the node weight w_nu = K sqrt(D2):
  *   Q[a] -= w Delta, Q[b] -= w Delta, Q[l],Q[m] += (1-r) w Delta,
- *   Q[ls],Q[ms] += r w Delta;   loss[a], loss[b] += w P.
- * K = 2 pi s0 n_box^2 d^4 / M for a differential cross-section s0 per
+ *   Q[ls],Q[ms] += r w Delta;   loss[a], loss[b] += 2 w P.
+ * Normalisation from the weak form: for a test function phi,
+ *   int Q phi = 1/2 int g f f1 (phi' + phi1' - phi - phi1)
+ *             = 1/4 int g (f f1 - f' f1') (phi' + phi1' - phi - phi1),
+ * the symmetric form counting every node's direct and inverse collision.  A
+ * rank-1 lattice of M nodes over (v, v1, w) in box x box x S^2 has volume
+ * weight V_box^2 4 pi s0 / M, so with V_box = n_box d^3 and g = d sqrt(D2)
+ * K = pi s0 n_box^2 d^4 / M for a differential cross-section s0 per
  * steradian (s0 = 1: the reference's 4 pi g total; s0 = 1/4: unit-diameter
- * spheres), D2 = |i_a - i_b|^2 in grid units.
+ * spheres); the loss f nu = 1/2 int g f f1 (...phi) takes twice that weight.
+ * D2 = |i_a - i_b|^2 in grid units.
  *
  * Integer geometry: with p = 2 i + 1 - n per axis the energy of a pair is the
  * integer sum of p^2 (spacing equal on all axes), so E0, E_lm, E_s and r are
@@ -801,7 +808,7 @@ void orc_korobov_collide(int npts, long nn, const int32_t *idx6,
                     R = (1.0 - rr) * Pl + rr * Ps;
             }
             val[2 * t] = w[t] * (P - R);
-            val[2 * t + 1] = w[t] * P;
+            val[2 * t + 1] = 2.0 * (w[t] * P);
         }
         for (int v = 0; v < npts; v++) L[v] = q[v] = 0.0;
         for (long t = 0; t < nn; t++) {

@@ -233,10 +233,12 @@ def test_korobov_steps_graph_equals_eager():
     assert np.max(np.abs(m1 - m0) / np.abs(m0).max(axis=0)) < 1e-13
 
 
-def test_korobov_fused_equals_two_phase():
-    """The fused LDS kernel and the node-value + reduce pair perform the same
-    operations per entry in the same order: identical Q and loss bits."""
-    say("korobov fused vs two-phase")
+def test_korobov_fused_equals_node_gather():
+    """The fused kernel (per-entry recomputation) and the node-value + grouped
+    gather compute the same per-entry terms (contraction off) and sum every
+    destination in different fixed orders: equal to FP64 rounding, and each
+    run-to-run reproducible bit for bit."""
+    say("korobov fused vs node+gather")
     vm = 4.8
     for n, M in (((20, 20, 20), 1 << 18), ((32, 32, 32), 1 << 19)):
         g = O.Grid(n, (-vm,) * 3, (vm,) * 3, vm)
@@ -247,16 +249,18 @@ def test_korobov_fused_equals_two_phase():
         f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
         fd = G.DeviceArray.from_host(f)
         nu1, g1, q1 = c.evaluate(G.METHOD_KOROBOV, fd, cells, 0)
-        nu2, g2, q2 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_TWO_PHASE)
-        assert np.array_equal(q1.get(), q2.get())
-        assert np.array_equal(nu1.get(), nu2.get())
+        nu2, g2, q2 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_FUSED)
+        assert rel_l2(q1.get(), q2.get()) < 1e-14
+        assert rel_l2(nu1.get(), nu2.get()) < 1e-14
+        _, _, q3 = c.evaluate(G.METHOD_KOROBOV, fd, cells, 0)
+        assert np.array_equal(q1.get(), q3.get())
         st = O.korobov_stencil(g, M, c.korobov_info()["a"], 0.25)
         ol, oq = O.korobov_collide(g, st, f)
         assert rel_l2(q1.get(), oq) < 1e-12
         # corrected: both paths conserve and agree to FP64 rounding
         _, _, c1 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_CORRECT)
-        _, _, c2 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_CORRECT | G.FLAG_TWO_PHASE)
-        assert rel_l2(c1.get(), c2.get()) < 1e-12
+        _, _, c2 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_CORRECT | G.FLAG_FUSED)
+        assert rel_l2(c1.get(), c2.get()) < 1e-13
         oqc, _, _ = O.correct_loss(g.xi, ol, oq, cells)
         assert rel_l2(c1.get(), oqc) < 1e-12
 
