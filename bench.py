@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roofline", action="store_true", help="skip the instrumented kernel-timing pass")
+    ap.add_argument("--fused", action="store_true", help="per-entry recomputation path (GDT_FLAG_FUSED)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -111,7 +112,7 @@ def main():
     f0 = mixture(ctx.xi, rng)
     fd = G.DeviceArray.from_host(f0[None])
 
-    flags = G.FLAG_CORRECT | G.FLAG_GRAPH
+    flags = G.FLAG_CORRECT | G.FLAG_GRAPH | (G.FLAG_FUSED if args.fused else 0)
     ctx.run_steps(G.METHOD_KOROBOV, fd, 1, DT, args.warmup, flags)
     ctx.sync()
     f_start = fd.get()[0]
@@ -129,24 +130,26 @@ def main():
     n = ctx.n_pnt
     nn, ne = info["nodes"], info["entries"]
     # algorithmic bytes per launch (cells = 1)
-    nparts = 256  # blocks per cell of the gather / fused kernels
+    G_ = 8        # node groups of the gather (one per XCD)
+    nparts = 512  # gather blocks per cell: 8 groups x 64 tiles at this size
     alg = {
         # node records (uint4 + D2) + f in; (D, 2wP) per node out
         "kor_node": nn * 20 + 8 * n + nn * 16,
-        # CSR entries (node|role + coefficient) + node values (each read at
-        # least once) + row pointers + centres in; Q, loss, moment sums out
-        "kor_gather": ne * 12 + nn * 16 + 4 * (n + 1) + 24 * n + 16 * n + nparts * 160,
-        # moment sums + centres + f + loss + Q in; Q and f out
-        "kor_apply": nparts * 160 + 24 * n + 24 * n + 16 * n,
+        # entry records (node|role, num|den) + node values (each read at least
+        # once) + group row pointers + centres in; group partials of Q and
+        # loss + moment sums out
+        "kor_gather": ne * 8 + nn * 16 + G_ * 4 * (n + 1) + 24 * n + G_ * 16 * n + nparts * 160,
+        # moment sums + centres + f + group partials in; f out
+        "kor_apply": nparts * 160 + 24 * n + 8 * n + G_ * 16 * n + 8 * n,
         # alternative fused kernel: 20-byte entry records + f + ptr + centres
-        "kor_fused": ne * 20 + 4 * (n + 1) + 8 * n + 24 * n + 16 * n + nparts * 160,
+        "kor_fused": ne * 20 + 4 * (n + 1) + 8 * n + 24 * n + 16 * n + 256 * 160,
     }
     roof = None
     kstats = {}
     if not args.no_roofline:
         ctx.kernel_stats_reset()
         reps = max(10, min(50, args.steps))
-        ctx.run_steps(G.METHOD_KOROBOV, fd, 1, DT, reps, G.FLAG_CORRECT | G.FLAG_TIME)
+        ctx.run_steps(G.METHOD_KOROBOV, fd, 1, DT, reps, (flags & ~G.FLAG_GRAPH) | G.FLAG_TIME)
         ctx.sync()
         for k in alg:
             ms, cnt = ctx.kernel_stats(k)

@@ -470,20 +470,17 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
         for (int v = 0; v < np; v++)
             for (int e = ptr[v]; e < ptr[v + 1]; e++) dest_of[e] = v;
         std::vector<uint32_t> e4(4 * (size_t)ne), e1(ne);
-        std::vector<double> coef(ne);
         for (long e = 0; e < ne; e++) {
             const uint32_t t = ent[e] >> 2, role = ent[e] & 3u;
             pack(t, &e4[4 * e]);
             e1[e] = (uint32_t)cint[3 * t + 2] | (role << 12) | ((uint32_t)dest_of[e] << 14);
-            coef[e] = role == 0 ? -1.0 : (role == 1 ? 1.0 - cr[t] : cr[t]);
         }
         GDT_CHECK(h->kn4.ensure(sizeof(uint32_t) * 4 * nn) == 0 && h->kn1.ensure(sizeof(uint32_t) * nn) == 0 &&
-                      h->kcoef.ensure(sizeof(double) * ne) == 0 && h->ke4.ensure(sizeof(uint32_t) * 4 * ne) == 0 &&
+                      h->ke4.ensure(sizeof(uint32_t) * 4 * ne) == 0 &&
                       h->ke1.ensure(sizeof(uint32_t) * ne) == 0,
                   GDT_E_NOMEM, "gdt_korobov_build: record allocation failed");
         GDT_HIP(hipMemcpy(h->kn4.p, n4.data(), sizeof(uint32_t) * 4 * nn, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->kn1.p, n1.data(), sizeof(uint32_t) * nn, hipMemcpyHostToDevice));
-        GDT_HIP(hipMemcpy(h->kcoef.p, coef.data(), sizeof(double) * ne, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke4.p, e4.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke1.p, e1.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
         // per node-group CSRs of the gather: group of node t is
@@ -502,21 +499,53 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 pg[v + 1] = (int32_t)run;
             }
         }
-        std::vector<uint32_t> gent(ne);
-        std::vector<double> gcoef(ne);
+        // entry records (node << 2 | role, num | den << 16 of the node)
+        std::vector<uint32_t> gent(2 * (size_t)ne);
         std::vector<int32_t> fill(gp);
         for (long e = 0; e < ne; e++) {  // ent is destination-sorted with (node, slot) order inside
             const int g = grp(ent[e] >> 2);
             const int32_t pos = fill[(size_t)g * (np + 1) + dest_of[e]]++;
-            gent[pos] = ent[e];
-            gcoef[pos] = coef[e];
+            gent[2 * (size_t)pos] = ent[e];
+            gent[2 * (size_t)pos + 1] = n4[4 * (size_t)(ent[e] >> 2) + 3];
         }
-        GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * gp.size()) == 0 && h->kgent.ensure(sizeof(uint32_t) * ne) == 0 &&
-                      h->kgcoef.ensure(sizeof(double) * ne) == 0,
+        // gather tiles: per group, contiguous destination ranges of at most
+        // KOR_TILE_DST destinations and about 1/NT of the group's entries; NT
+        // is the largest count (<= 64) for which every group needs at most 64
+        // tiles, so that G * 64 blocks, two per CU, fill the chip in one round
+        std::vector<int4> tiles;
+        int ntile = 0, maxe = 0;
+        for (int NT = 64; NT >= 1; NT--) {
+            std::vector<std::vector<int4>> per(G);
+            ntile = 0;
+            maxe = 0;
+            for (int g = 0; g < G; g++) {
+                const int32_t *pg = &gp[(size_t)g * (np + 1)];
+                const long quota = std::max(1L, ((long)pg[np] - pg[0] + NT - 1) / NT);
+                int d = 0;
+                while (d < np) {
+                    int e = d;
+                    while (e < np && e - d < KOR_TILE_DST && (e == d || (long)pg[e] - pg[d] < quota)) e++;
+                    per[g].push_back(make_int4(d, e, pg[d], pg[e]));
+                    maxe = std::max(maxe, pg[e] - pg[d]);
+                    d = e;
+                }
+                ntile = std::max(ntile, (int)per[g].size());
+            }
+            if (ntile <= 64 || NT == 1) {
+                tiles.assign((size_t)G * ntile, make_int4(0, 0, 0, 0));
+                for (int g = 0; g < G; g++)
+                    for (size_t k = 0; k < per[g].size(); k++) tiles[(size_t)g * ntile + k] = per[g][k];
+                break;
+            }
+        }
+        GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * gp.size()) == 0 && h->kgent.ensure(sizeof(uint32_t) * 2 * ne) == 0 &&
+                      h->ktile.ensure(sizeof(int4) * tiles.size()) == 0,
                   GDT_E_NOMEM, "gdt_korobov_build: group CSR allocation failed");
         GDT_HIP(hipMemcpy(h->kgptr.p, gp.data(), sizeof(int32_t) * gp.size(), hipMemcpyHostToDevice));
-        GDT_HIP(hipMemcpy(h->kgent.p, gent.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
-        GDT_HIP(hipMemcpy(h->kgcoef.p, gcoef.data(), sizeof(double) * ne, hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kgent.p, gent.data(), sizeof(uint32_t) * 2 * ne, hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->ktile.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+        h->kntile = ntile;
+        h->kcap = std::max(64, std::min(maxe, KOR_TILE_CAP));
         int rc2 = kor_setup(h, 1);
         if (rc2) return rc2;
         h->have_stream = true;
@@ -800,8 +829,8 @@ static int one_step(gdt_context *h, int method, double *f, int cells, double dt,
     if (method == GDT_METHOD_KOROBOV) {
         KorPartials kp;
         if ((rc = kor_partials(h, f, cells, flags, kp))) return rc;
-        return launch_kor_apply(h, corr, kp.G, kp.nparts, h->s_mpart.as<double>(), f, kp.L, kp.Q,
-                                h->s_Q.as<double>(), nullptr, nullptr, nullptr, cells, dt, f);
+        return launch_kor_apply(h, corr, kp.G, kp.nparts, h->s_mpart.as<double>(), f, kp.L, kp.Q, nullptr,
+                                nullptr, nullptr, nullptr, cells, dt, f);
     }
     double *nu = h->s_nu.as<double>(), *g = h->s_out2.as<double>(), *Q = h->s_Q.as<double>();
     if ((rc = collide_matrix_cm(h, f, nu, g, cells))) return rc;

@@ -67,12 +67,14 @@ struct gdt_context {
     double ks0 = 0.0, kK = 0.0;
     long nn = 0, nent = 0;
     DevBuf kidx, kr, kw, knode, kptr, kent;
-    // per-node records (uint4 index pairs + num | den, uint32 D2), per-entry
-    // coefficients (-1, 1-r, r) and, for the fused path, per-entry records
+    // per-node records (uint4 index pairs + num | den, uint32 D2) and, for the
+    // fused path, per-entry records
     // (uint4 as the node's, uint32 D2 | role << 12 | destination << 14)
     bool have_stream = false;
-    DevBuf kn4, kn1, kcoef, ke4, ke1, kwb;
-    DevBuf kgptr, kgent, kgcoef;  // per node-group CSRs of the gather
+    DevBuf kn4, kn1, ke4, ke1, kwb;
+    DevBuf kgptr, kgent;          // per node-group CSRs of the gather (uint2 entry records)
+    DevBuf ktile;                 // gather tiles (d0, d1, e0, e1), [group][kntile]
+    int kntile = 0, kcap = 0;     // tiles per group, LDS entries per gather pass
     std::vector<int32_t> h_kptr;  // host copy of the CSR row pointers
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
     int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
@@ -118,7 +120,9 @@ int kor_setup(gdt_context *h, int cells);
 int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells);
 int kor_gather_blocks(gdt_context *h, int cells);
 int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Qpart, double *mpart, int cells);
-#define KOR_NGROUPS 8
+#define KOR_NGROUPS 8     // node groups of the gather: one per XCD
+#define KOR_TILE_DST 512  // destinations per gather tile: one per thread
+#define KOR_TILE_CAP 4864 // entries per gather pass: 76 KB of LDS, two blocks per CU
 int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells);
 // G group partials [cell][G][node] of Q and loss (G = 1: plain arrays)
 int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const double *mpart, const double *f,

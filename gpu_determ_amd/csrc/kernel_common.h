@@ -15,6 +15,26 @@ __device__ __forceinline__ double dpp_f64(double v)
     return __hiloint2double(hi, lo);
 }
 
+// DPP exchange that leaves `old` in lanes without a source lane (row_shr at
+// the row start, wave_shl at lane 63) and in rows outside ROWMASK.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i32_or(int v, int old)
+{
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWMASK, 0xF, false);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64_or(double v, double old)
+{
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+#define DPP_ROW_SHR(k) (0x110 + (k))  // lane i of a row reads lane i - k of the row
+#define DPP_WAVE_SHL1 0x130           // lane i reads lane i + 1
+#define DPP_ROW_BCAST15 0x142         // lane 15 of each row feeds the next row
+#define DPP_ROW_BCAST31 0x143         // lane 31 feeds rows 2 and 3
 #define DPP_QUAD_XOR1 0xB1       // quad_perm [1,0,3,2]
 #define DPP_QUAD_XOR2 0x4E       // quad_perm [2,3,0,1]
 #define DPP_ROW_HALF_MIRROR 0x141
@@ -29,6 +49,34 @@ __device__ __forceinline__ double row16_sum(double v)
     v += dpp_f64<DPP_ROW_HALF_MIRROR>(v);
     v += dpp_f64<DPP_ROW_MIRROR>(v);
     return v;
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_step(double &q, double &l, int key)
+{
+    const int ks = dpp_i32_or<CTRL, ROWMASK>(key, -1);
+    const double qs = dpp_f64_or<CTRL, ROWMASK>(q, 0.0);
+    const double ls = dpp_f64_or<CTRL, ROWMASK>(l, 0.0);
+    if (ks == key) {
+        q += qs;
+        l += ls;
+    }
+}
+
+// Inclusive segmented scan of (q, l) over the 64 lanes: afterwards every lane
+// holds the sum over the lanes up to it that share its key.  Keys of valid
+// lanes are >= 0 and non-decreasing with the lane (runs are contiguous);
+// other lanes carry distinct keys < -1.  Row steps (shift 1, 2, 4, 8) and the
+// two cross-row broadcasts are DPP modifiers (no LDS traffic); the pairing
+// order is fixed, so sums are reproducible.
+__device__ __forceinline__ void seg_scan_dpp(double &q, double &l, int key)
+{
+    seg_step<DPP_ROW_SHR(1), 0xF>(q, l, key);
+    seg_step<DPP_ROW_SHR(2), 0xF>(q, l, key);
+    seg_step<DPP_ROW_SHR(4), 0xF>(q, l, key);
+    seg_step<DPP_ROW_SHR(8), 0xF>(q, l, key);
+    seg_step<DPP_ROW_BCAST15, 0xA>(q, l, key);
+    seg_step<DPP_ROW_BCAST31, 0xC>(q, l, key);
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int lane)
@@ -186,10 +234,10 @@ __device__ __forceinline__ void moments_solve(const double (&s)[20], double (&X)
     solve5_spd(A, B, X);
 }
 
-// Adds the moment contributions of destination g (Q value q, loss value l).
-__device__ __forceinline__ void moments_add(double (&m)[20], const double *xi, int g, double q, double l)
+// Adds the moment contributions of a destination at velocity (x0, y0, z0)
+// (Q value q, loss value l).
+__device__ __forceinline__ void moments_add_psi(double (&m)[20], double x0, double y0, double z0, double q, double l)
 {
-    const double x0 = xi[3 * g], y0 = xi[3 * g + 1], z0 = xi[3 * g + 2];
     const double psi[5] = {1.0, x0, y0, z0, x0 * x0 + y0 * y0 + z0 * z0};
 #pragma unroll
     for (int a = 0; a < 5; a++) m[a] += q * psi[a];
@@ -198,4 +246,10 @@ __device__ __forceinline__ void moments_add(double (&m)[20], const double *xi, i
     for (int a = 0; a < 5; a++)
 #pragma unroll
         for (int b = a; b < 5; b++) m[k++] += l * psi[a] * psi[b];
+}
+
+// moments_add_psi for destination g of the node list xi.
+__device__ __forceinline__ void moments_add(double (&m)[20], const double *xi, int g, double q, double l)
+{
+    moments_add_psi(m, xi[3 * g], xi[3 * g + 1], xi[3 * g + 2], q, l);
 }

@@ -36,7 +36,7 @@
 
 #define KOR_THREADS 1024
 #define KOR_WAVES (KOR_THREADS / WAVE)
-#define KOR_GROUPS 8  // node groups of the gather: one per XCD
+#define KOR_GROUPS KOR_NGROUPS
 
 // D and 2 w P of one node from the LDS copy of f.  x = (a | b << 16,
 // l | m << 16, ls | ms << 16, num | den << 16), D2 in the low 12 bits of y.
@@ -64,38 +64,62 @@ __device__ __forceinline__ void node_terms(const uint4 x, const uint32_t y, cons
     W2P = 2.0 * (w * P);
 }
 
-// Stages f of cell c into LDS with 8 loads in flight per thread.
+// Stages f of cell c into LDS.  Every load of the thread is issued before the
+// first store, so the staging costs one memory round trip (npts <=
+// KOR_STAGE * KOR_THREADS = 19,456; the stencil build admits at most 19,200).
+#define KOR_STAGE 19
 __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts)
 {
-    const int per = (npts + KOR_THREADS - 1) / KOR_THREADS;
-    for (int k0 = 0; k0 < per; k0 += 8) {
-        double v[8];
+    double v[KOR_STAGE];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int t = (k0 + j) * KOR_THREADS + threadIdx.x;
-            v[j] = t < npts ? fc[t] : 0.0;
-        }
+    for (int j = 0; j < KOR_STAGE; j++) {
+        const int t = j * KOR_THREADS + threadIdx.x;
+        v[j] = fc[t < npts ? t : npts - 1];  // unconditional (clamped) loads: no branches
+    }
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int t = (k0 + j) * KOR_THREADS + threadIdx.x;
-            if (t < npts) fs[t] = v[j];
-        }
+    for (int j = 0; j < KOR_STAGE; j++) {
+        const int t = j * KOR_THREADS + threadIdx.x;
+        if (t < npts) fs[t] = v[j];
     }
 }
 
 // ---------------------------------------------------------------------------
 // node values
 // ---------------------------------------------------------------------------
+// Per node t of cell c, the values the destination entries read:
+//   V[t] = (D, 2 w P)
+// A thread owns nodes t and t + stride and evaluates both together, so the two
+// dependency chains (LDS reads, log / exp) overlap; their records are loaded
+// before the staging of f, so that latency overlaps too.  Further nodes (more
+// than two per thread) are handled one at a time.
 __global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, const uint4 *__restrict__ nrec,
                                                            const uint32_t *__restrict__ nd2, double K,
-                                                           const double *__restrict__ f, double2 *__restrict__ DW)
+                                                           const double *__restrict__ f, double2 *__restrict__ V)
 {
+#pragma clang fp contract(off)
     extern __shared__ double fs[];
     const int c = blockIdx.y;
+    const long stride = (long)gridDim.x * KOR_THREADS;
+    const long ta = (long)blockIdx.x * KOR_THREADS + threadIdx.x, tb = ta + stride;
+    uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
+    uint32_t ya = 0u, yb = 0u;
+    if (ta < nn) {
+        xa = nrec[ta];
+        ya = nd2[ta];
+    }
+    if (tb < nn) {
+        xb = nrec[tb];
+        yb = nd2[tb];
+    }
     stage_f(fs, f + (size_t)c * npts, npts);
     __syncthreads();
-    double2 *out = DW + (size_t)c * nn;
-    for (long t = (long)blockIdx.x * KOR_THREADS + threadIdx.x; t < nn; t += (long)gridDim.x * KOR_THREADS) {
+    double2 *out = V + (size_t)c * nn;
+    double Da, Wa, ra, Db, Wb, rb;
+    node_terms(xa, ya, fs, K, Da, Wa, ra);
+    node_terms(xb, yb, fs, K, Db, Wb, rb);
+    if (ta < nn) out[ta] = make_double2(Da, Wa);
+    if (tb < nn) out[tb] = make_double2(Db, Wb);
+    for (long t = tb + stride; t < nn; t += stride) {
         double D, W2P, r;
         node_terms(nrec[t], nd2[t], fs, K, D, W2P, r);
         out[t] = make_double2(D, W2P);
@@ -103,82 +127,28 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, con
 }
 
 // ---------------------------------------------------------------------------
-// segmented reduction helpers
+// block helpers
 // ---------------------------------------------------------------------------
-// Inclusive scan of (q, l) over lanes whose keys are equal; keys form
-// contiguous runs, so a lane off positions down with the same key is in the
-// same run.
-__device__ __forceinline__ void seg_scan(double &q, double &l, int key, int lane)
+// Block sum of the 20 moment sums (DPP row sums, then the rows in a fixed
+// order), written to mpart[cell][moment][block].
+template <int NT>
+__device__ __forceinline__ void block_moments(double (&m)[20], double (*rpart)[20], double *mpart, int c)
 {
 #pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        const double qo = __shfl_up(q, off, WAVE);
-        const double lo = __shfl_up(l, off, WAVE);
-        const int ko = __shfl_up(key, off, WAVE);
-        if (lane >= off && ko == key) {
-            q += qo;
-            l += lo;
-        }
-    }
-}
-
-// After a chunk's scan: the tail lane of every run holds the run's sum.
-// Runs that end inside the chunk are complete (joined with the carry if they
-// continue the previous chunk's open run); the chunk's last run stays open
-// unless the wave's range ends here.  Completed destinations are written and
-// added to the lane's moment sums.
-struct SegState {
-    int ckey = -1;
-    double cq = 0.0, cl = 0.0;
-};
-
-__device__ __forceinline__ void seg_flush_carry(SegState &st, int kfirst, int lane, double *Qc, double *Lc,
-                                                const double *xi, double (&m)[20])
-{
-    if (st.ckey >= 0 && kfirst != st.ckey) {
-        if (lane == 0) {
-            Qc[st.ckey] = st.cq;
-            if (Lc) Lc[st.ckey] = st.cl;
-            moments_add(m, xi, st.ckey, st.cq, st.cl);
-        }
-        st.ckey = -1;
-    }
-}
-
-__device__ __forceinline__ void seg_emit(SegState &st, double q, double l, int key, bool valid, int nvalid,
-                                         bool last_chunk, int lane, double *Qc, double *Lc, const double *xi,
-                                         double (&m)[20])
-{
-    const int knext = __shfl_down(key, 1, WAVE);
-    const bool tail = valid && (lane == nvalid - 1 || knext != key);
-    const int klast = __shfl(key, nvalid - 1, WAVE);
-    if (tail) {
-        if (key == st.ckey) {
-            q = st.cq + q;
-            l = st.cl + l;
-        }
-        if (key != klast || last_chunk) {
-            Qc[key] = q;
-            if (Lc) Lc[key] = l;
-            moments_add(m, xi, key, q, l);
-        }
-    }
-    st.cq = __shfl(q, nvalid - 1, WAVE);
-    st.cl = __shfl(l, nvalid - 1, WAVE);
-    st.ckey = last_chunk ? -1 : klast;
-}
-
-__device__ __forceinline__ void block_moments(double (&m)[20], double (*wpart)[20], double *mpart, int c)
-{
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    wave_sum_n<20>(m);
-    if (lane == 0)
+    for (int i = 0; i < 20; i++) m[i] += dpp_f64<DPP_QUAD_XOR1>(m[i]);
 #pragma unroll
-        for (int q = 0; q < 20; q++) wpart[wave][q] = m[q];
+    for (int i = 0; i < 20; i++) m[i] += dpp_f64<DPP_QUAD_XOR2>(m[i]);
+#pragma unroll
+    for (int i = 0; i < 20; i++) m[i] += dpp_f64<DPP_ROW_HALF_MIRROR>(m[i]);
+#pragma unroll
+    for (int i = 0; i < 20; i++) m[i] += dpp_f64<DPP_ROW_MIRROR>(m[i]);
+    if ((threadIdx.x & 15) == 0)
+#pragma unroll
+        for (int i = 0; i < 20; i++) rpart[threadIdx.x / 16][i] = m[i];
     __syncthreads();
     if (threadIdx.x < 20) {
         double sm = 0.0;
-        for (int w = 0; w < KOR_WAVES; w++) sm += wpart[w][threadIdx.x];
+        for (int r = 0; r < NT / 16; r++) sm += rpart[r][threadIdx.x];
         mpart[((size_t)c * 20 + threadIdx.x) * gridDim.x + blockIdx.x] = sm;
     }
 }
@@ -194,84 +164,130 @@ __device__ __forceinline__ void zero_empty(int ga, int gb, const int32_t *ptr, d
 }
 
 // ---------------------------------------------------------------------------
-// destination-sorted gather, one node group per XCD
+// destination-sorted gather in tiles, one node group per XCD
 // ---------------------------------------------------------------------------
-#define QW 16  // lanes per destination (one DPP row)
+// Entry record: (node << 2 | role, num | den << 16 of the node).  An entry's
+// term is, from the node's (D, 2 w P) and r = num / den (the node kernel's
+// operations, so the terms are bit-identical to the oracle's):
+//   role 0 (a, b):   q = -D,  l = 2 w P;   role 1 (l, m): q = (1 - r) D;
+//   role 2 (ls, ms): q = r D.
+// Tile (d0, d1, e0, e1) of group g: destinations [d0, d1) of the group's CSR,
+// whose entries are [e0, e1).  Block b serves group b % KOR_GROUPS (blocks are
+// dealt to the XCDs round robin, so a group's node values stay in one L2).
+// Per pass over at most `cap` entries:
+//   phase 1  every thread issues the entry loads of its batch, then the node
+//            value loads, and parks (q, l) of each entry in LDS by entry
+//            position -- no dependent chains beyond entry -> value;
+//   phase 2  thread i owns destinations d0 + i + j * TILE_THREADS and adds
+//            their entries of the pass in entry order (fixed order:
+//            reproducible sums).
+// The destinations' row pointers and coordinates are loaded before phase 1.
+// Two blocks fit a CU (LDS), so the G * 64 blocks of a cell run in one round.
+#define TILE_THREADS 256
+#define TILE_WAVES (TILE_THREADS / WAVE)
+#define TILE_DPT (KOR_TILE_DST / TILE_THREADS)  // destinations per thread
+#define TILE_BATCH 20
 
-__global__ __launch_bounds__(KOR_THREADS) void k_kor_gather(int npts, long nn, int dps,
-                                                             const int32_t *__restrict__ gptr,
-                                                             const uint32_t *__restrict__ ent,
-                                                             const double *__restrict__ coef,
-                                                             const double2 *__restrict__ DW,
-                                                             const double *__restrict__ xi,
-                                                             double *__restrict__ Lpart, double *__restrict__ Qpart,
-                                                             double *__restrict__ mpart)
+__global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, int ntile, int cap,
+                                                              const int4 *__restrict__ tiles,
+                                                              const int32_t *__restrict__ gptr,
+                                                              const uint2 *__restrict__ ent,
+                                                              const double2 *__restrict__ V,
+                                                              const double *__restrict__ xi,
+                                                              double *__restrict__ Lpart, double *__restrict__ Qpart,
+                                                              double *__restrict__ mpart)
 {
 #pragma clang fp contract(off)
-    __shared__ double wpart[KOR_WAVES][20];
+    extern __shared__ double2 ql[];
+    __shared__ double rpart[TILE_WAVES * 4][20];
     const int c = blockIdx.y;
-    const int g = blockIdx.x % KOR_GROUPS, sub = blockIdx.x / KOR_GROUPS;
-    const int d0 = sub * dps, d1 = min(npts, d0 + dps);
+    const int g = blockIdx.x % KOR_GROUPS, k = blockIdx.x / KOR_GROUPS;
+    const int4 tl = tiles[(size_t)g * ntile + k];
+    const int d0 = tl.x, d1 = tl.y, e0 = tl.z, e1 = tl.w;
     const int32_t *ptr = gptr + (size_t)g * (npts + 1);
-    const double2 *dw = DW + (size_t)c * nn;
-    const size_t po = ((size_t)c * KOR_GROUPS + g) * npts;
-    const int lane = threadIdx.x % WAVE, ql = threadIdx.x % QW, qwid = threadIdx.x / QW;
-    double m[20];
+    int ea[TILE_DPT], eb[TILE_DPT];
+    double px[TILE_DPT], py[TILE_DPT], pz[TILE_DPT], q[TILE_DPT], l[TILE_DPT];
 #pragma unroll
-    for (int q = 0; q < 20; q++) m[q] = 0.0;
-    const int step = KOR_THREADS / QW;
-    int dst = d0 + qwid;
-    int e0 = dst < d1 ? ptr[dst] : 0, e1 = dst < d1 ? ptr[dst + 1] : 0;
-    for (; dst < d1; dst += step) {
-        // next destination's range, loaded ahead
-        const int nd = dst + step;
-        const int ne0 = nd < d1 ? ptr[nd] : 0, ne1 = nd < d1 ? ptr[nd + 1] : 0;
-        double q = 0.0, l = 0.0;
-        for (int e = e0 + ql; e < e1; e += 2 * QW) {
-            const bool two = e + QW < e1;
-            const uint32_t xa = ent[e];
-            const uint32_t xb = two ? ent[e + QW] : 0u;
-            const double ca = coef[e];
-            const double cb = two ? coef[e + QW] : 0.0;
-            const double2 va = dw[xa >> 2];
-            const double2 vb = two ? dw[xb >> 2] : make_double2(0.0, 0.0);
-            const double ta = ca * va.x;
-            q += ta;
-            if ((xa & 3u) == 0) l += va.y;
-            if (two) {
-                const double tb = cb * vb.x;
-                q += tb;
-                if ((xb & 3u) == 0) l += vb.y;
+    for (int i = 0; i < TILE_DPT; i++) {
+        const int d = d0 + threadIdx.x + i * TILE_THREADS;
+        ea[i] = eb[i] = 0;
+        px[i] = py[i] = pz[i] = q[i] = l[i] = 0.0;
+        if (d < d1) {
+            ea[i] = ptr[d];
+            eb[i] = ptr[d + 1];
+            px[i] = xi[3 * d];
+            py[i] = xi[3 * d + 1];
+            pz[i] = xi[3 * d + 2];
+        }
+    }
+    const double2 *Vc = V + (size_t)c * nn;
+    for (int w0 = e0; w0 < e1; w0 += cap) {
+        const int w1 = min(e1, w0 + cap);
+        if (w0 > e0) __syncthreads();  // the previous pass's reads are done
+        for (int j0 = w0 + threadIdx.x; j0 < w1; j0 += TILE_BATCH * TILE_THREADS) {
+            uint2 x[TILE_BATCH];
+#pragma unroll
+            for (int j = 0; j < TILE_BATCH; j++) {
+                const int e = j0 + j * TILE_THREADS;
+                x[j] = e < w1 ? ent[e] : make_uint2(0u, 0u);
+            }
+            double2 v[TILE_BATCH];
+#pragma unroll
+            for (int j = 0; j < TILE_BATCH; j++) {
+                const int e = j0 + j * TILE_THREADS;
+                v[j] = e < w1 ? Vc[x[j].x >> 2] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int j = 0; j < TILE_BATCH; j++) {
+                const int e = j0 + j * TILE_THREADS;
+                if (e < w1) {
+                    const uint32_t role = x[j].x & 3u;
+                    const int num = (int)(short)(x[j].y & 0xFFFFu), den = (int)(short)(x[j].y >> 16);
+                    const double r = num == 0 ? 0.0 : (double)num / (double)den;
+                    ql[e - w0] = role == 0u ? make_double2(-1.0 * v[j].x, v[j].y)
+                                            : make_double2((role == 1u ? (1.0 - r) : r) * v[j].x, 0.0);
+                }
             }
         }
-        q = row16_sum(q);
-        l = row16_sum(l);
-        if (ql == 0) {
-            Qpart[po + dst] = q;
-            Lpart[po + dst] = l;
-            moments_add(m, xi, dst, q, l);
-        }
-        e0 = ne0;
-        e1 = ne1;
-    }
-    const int wave = threadIdx.x / WAVE;
-    wave_sum_n<20>(m);
-    if (lane == 0)
+        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 20; q++) wpart[wave][q] = m[q];
-    __syncthreads();
-    if (threadIdx.x < 20) {
-        double sm = 0.0;
-        for (int w = 0; w < KOR_WAVES; w++) sm += wpart[w][threadIdx.x];
-        mpart[((size_t)c * 20 + threadIdx.x) * gridDim.x + blockIdx.x] = sm;
+        for (int i = 0; i < TILE_DPT; i++) {
+            const int a = max(ea[i], w0), b = min(eb[i], w1);
+            for (int e = a; e < b; e++) {
+                const double2 t = ql[e - w0];
+                q[i] += t.x;
+                l[i] += t.y;
+            }
+        }
     }
+    double m[20];
+#pragma unroll
+    for (int i = 0; i < 20; i++) m[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < TILE_DPT; i++) {
+        const int d = d0 + threadIdx.x + i * TILE_THREADS;
+        if (d < d1) {
+            const size_t po = ((size_t)c * KOR_GROUPS + g) * npts + d;
+            Qpart[po] = q[i];
+            Lpart[po] = l[i];
+            moments_add_psi(m, px[i], py[i], pz[i], q[i], l[i]);
+        }
+    }
+    block_moments<TILE_THREADS>(m, rpart, mpart, c);
 }
 
 // ---------------------------------------------------------------------------
-// alternative: fused, per-entry recomputation
+// fused: per-entry recomputation from the LDS copy of f
 // ---------------------------------------------------------------------------
-// One 20-byte record per CSR entry: the node's six indices, num | den, and
-// D2 | role << 12 | destination << 14.
+// One 20-byte record per entry of the global destination-sorted CSR: the
+// node's six indices and num | den (as the node record) and D2 | role << 12 |
+// destination << 14.  Wave w of the grid owns the destinations
+// [wbound[w], wbound[w+1]) (whole destinations, balanced entry counts) and
+// walks their entries in chunks of 64: every lane evaluates its entry's term
+// from f in LDS, a DPP segmented scan sums the chunk's runs of equal
+// destination, and the run tails write Q and loss (a run that reaches the
+// chunk's end and continues in the next chunk is carried).  Records are
+// loaded two chunks ahead; the first two are in flight during the staging.
 __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32_t *__restrict__ wbound,
                                                             const int32_t *__restrict__ ptr,
                                                             const uint4 *__restrict__ e4,
@@ -283,75 +299,101 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 {
 #pragma clang fp contract(off)
     extern __shared__ double fs[];
-    __shared__ double wpart[KOR_WAVES][20];
+    __shared__ double rpart[KOR_THREADS / 16][20];
     const int c = blockIdx.y;
-    stage_f(fs, f + (size_t)c * npts, npts);
-    __syncthreads();
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int wg = blockIdx.x * KOR_WAVES + wave;
     const int ga = wbound[wg], gb = wbound[wg + 1];
     const int ea = ptr[ga], eb = ptr[gb];
+    uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
+    uint32_t ya = 0u, yb = 0u;
+    if (ea + lane < eb) {
+        xa = e4[ea + lane];
+        ya = e1[ea + lane];
+    }
+    if (ea + WAVE + lane < eb) {
+        xb = e4[ea + WAVE + lane];
+        yb = e1[ea + WAVE + lane];
+    }
+    stage_f(fs, f + (size_t)c * npts, npts);
+    __syncthreads();
     double *Qc = Q + (size_t)c * npts;
-    double *Lc = loss ? loss + (size_t)c * npts : nullptr;
+    double *Lc = loss + (size_t)c * npts;
     double m[20];
 #pragma unroll
-    for (int q = 0; q < 20; q++) m[q] = 0.0;
+    for (int i = 0; i < 20; i++) m[i] = 0.0;
     zero_empty(ga, gb, ptr, Qc, Lc, lane);
 
-    SegState st;
-    uint4 xn = make_uint4(0, 0, 0, 0);
-    uint32_t yn = 0;
-    if (ea + lane < eb) {
-        xn = e4[ea + lane];
-        yn = e1[ea + lane];
-    }
+    int ckey = -1;  // open run carried from the previous chunk
+    double cq = 0.0, cl = 0.0;
     for (int base = ea; base < eb; base += WAVE) {
-        const uint4 x = xn;
-        const uint32_t y = yn;
-        const int e = base + lane;
-        const bool valid = e < eb;
-        if (base + WAVE + lane < eb) {
-            xn = e4[base + WAVE + lane];
-            yn = e1[base + WAVE + lane];
+        const uint4 x = xa;
+        const uint32_t y = ya;
+        xa = xb;
+        ya = yb;
+        const int e2 = base + 2 * WAVE + lane;
+        if (e2 < eb) {
+            xb = e4[e2];
+            yb = e1[e2];
         }
+        const bool valid = base + lane < eb;
         double q = 0.0, l = 0.0;
         int key = -2 - lane;
         if (valid) {
             double D, W2P, r;
             node_terms(x, y, fs, K, D, W2P, r);
             const uint32_t role = (y >> 12) & 3u;
-            if (role == 0) {
+            if (role == 0u) {
                 q = -1.0 * D;
                 l = W2P;
             } else {
-                q = (role == 1 ? (1.0 - r) : r) * D;
+                q = (role == 1u ? (1.0 - r) : r) * D;
             }
             key = (int)(y >> 14);
         }
-        seg_flush_carry(st, __shfl(key, 0, WAVE), lane, Qc, Lc, xi, m);
-        seg_scan(q, l, key, lane);
-        const int nvalid = min(WAVE, eb - base);
-        seg_emit(st, q, l, key, valid, nvalid, base + WAVE >= eb, lane, Qc, Lc, xi, m);
+        seg_scan_dpp(q, l, key);
+        if (key == ckey) {
+            q += cq;
+            l += cl;
+        }
+        // a run ends where the next lane's key differs; lane 63 compares with
+        // the first entry of the next chunk
+        const bool more = base + WAVE < eb;
+        const int next_first = more ? (int)(__builtin_amdgcn_readfirstlane(ya) >> 14) : -1;
+        int knext = dpp_i32_or<DPP_WAVE_SHL1, 0xF>(key, -1);
+        if (lane == WAVE - 1) knext = next_first;
+        if (valid && knext != key) {
+            Qc[key] = q;
+            Lc[key] = l;
+            moments_add(m, xi, key, q, l);
+        }
+        if (more) {
+            const int k63 = __builtin_amdgcn_readlane(key, WAVE - 1);
+            ckey = k63 == next_first ? k63 : -1;
+            cq = readlane_f64(q, WAVE - 1);
+            cl = readlane_f64(l, WAVE - 1);
+        }
     }
-    block_moments(m, wpart, mpart, c);
+    block_moments<KOR_THREADS>(m, rpart, mpart, c);
 }
 
 // ---------------------------------------------------------------------------
 // group sum, correction from the moment sums (+ explicit step)
 // ---------------------------------------------------------------------------
 // Every block reduces its cell's nparts x 20 moment sums in the same order
-// (all loads issued at once) and solves the same 5x5 system; then each
-// thread sums its node's G group partials in group order and writes
+// and solves the same 5x5 system; then each thread sums its node's G group
+// partials in group order and writes
 //   Q = Q0 - L X.psi,  loss = L,  nu = (L/f)(1 + X.psi),  f <- f + dt Q.
-template <bool CORR>
-__global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, int G, const double *__restrict__ mpart,
+#define APPLY_ROUNDS 2  // block sums per thread and moment with loads in flight (512 blocks: one round)
+template <bool CORR, int G>
+__global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, const double *__restrict__ mpart,
                                                    const double *__restrict__ xi, const double *f,
                                                    const double *__restrict__ Lpart,
                                                    const double *__restrict__ Qpart, double *__restrict__ Qout,
                                                    double *__restrict__ Lout, double *__restrict__ nu_out,
                                                    double *__restrict__ Xout, double dt, double *f_upd)
 {
-    __shared__ double part[4][20];
+    __shared__ double red[16][20];
     __shared__ double Xs[5];
     const int c = blockIdx.y;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -362,32 +404,58 @@ __global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, int G, 
         y = xi[3 * v + 1];
         z = xi[3 * v + 2];
         fv = f[o];
+        double qg[G], lg[G];
+#pragma unroll
         for (int g = 0; g < G; g++) {
             const size_t po = ((size_t)c * G + g) * npts + v;
-            q0 += Qpart[po];
-            L += Lpart[po];
+            qg[g] = Qpart[po];
+            lg[g] = Lpart[po];
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            q0 += qg[g];
+            L += lg[g];
         }
     }
     if (CORR) {
+        // block sums of the moments: thread t adds blocks t, t + 256, ...
+        // (coalesced, all loads in flight), then DPP row sums and the 16
+        // row sums in a fixed order (every block gets the same X)
         double m[20];
 #pragma unroll
-        for (int q = 0; q < 20; q++) m[q] = 0.0;
-        for (int p0 = 0; p0 < nparts; p0 += blockDim.x) {
-            const int p = p0 + threadIdx.x;
-            if (p < nparts) {
+        for (int a = 0; a < 20; a++) m[a] = 0.0;
+        for (int p0 = 0; p0 < nparts; p0 += APPLY_ROUNDS * 256) {
+            double vv[APPLY_ROUNDS][20];
 #pragma unroll
-                for (int q = 0; q < 20; q++) m[q] += mpart[((size_t)c * 20 + q) * nparts + p];
+            for (int k = 0; k < APPLY_ROUNDS; k++) {
+                const int p = p0 + k * 256 + threadIdx.x;
+#pragma unroll
+                for (int a = 0; a < 20; a++) vv[k][a] = p < nparts ? mpart[((size_t)c * 20 + a) * nparts + p] : 0.0;
             }
-        }
-        const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-        wave_sum_n<20>(m);
-        if (lane == 0)
 #pragma unroll
-            for (int q = 0; q < 20; q++) part[wave][q] = m[q];
+            for (int k = 0; k < APPLY_ROUNDS; k++)
+#pragma unroll
+                for (int a = 0; a < 20; a++) m[a] += vv[k][a];
+        }
+#pragma unroll
+        for (int a = 0; a < 20; a++) m[a] += dpp_f64<DPP_QUAD_XOR1>(m[a]);
+#pragma unroll
+        for (int a = 0; a < 20; a++) m[a] += dpp_f64<DPP_QUAD_XOR2>(m[a]);
+#pragma unroll
+        for (int a = 0; a < 20; a++) m[a] += dpp_f64<DPP_ROW_HALF_MIRROR>(m[a]);
+#pragma unroll
+        for (int a = 0; a < 20; a++) m[a] += dpp_f64<DPP_ROW_MIRROR>(m[a]);
+        if ((threadIdx.x & 15) == 0)
+#pragma unroll
+            for (int a = 0; a < 20; a++) red[threadIdx.x / 16][a] = m[a];
         __syncthreads();
         if (threadIdx.x == 0) {
             double sm[20], X[5];
-            for (int q = 0; q < 20; q++) sm[q] = part[0][q] + part[1][q] + part[2][q] + part[3][q];
+            for (int a = 0; a < 20; a++) {
+                double t = 0.0;
+                for (int r = 0; r < 16; r++) t += red[r][a];
+                sm[a] = t;
+            }
             moments_solve(sm, X);
             for (int a = 0; a < 5; a++) Xs[a] = X[a];
         }
@@ -444,6 +512,8 @@ int kor_setup(gdt_context *h, int cells)
     if (h->lds_attr < lds) {
         GDT_HIP(hipFuncSetAttribute((const void *)k_kor_node, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
         GDT_HIP(hipFuncSetAttribute((const void *)k_kor_fused, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        GDT_HIP(hipFuncSetAttribute((const void *)k_kor_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(double2) * KOR_TILE_CAP)));
         h->lds_attr = lds;
     }
     return kor_wave_bounds(h, kor_blocks(h, cells));
@@ -461,19 +531,17 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
 
 int kor_gather_blocks(gdt_context *h, int cells)
 {
-    // per group: about 256 blocks per cell in total (one round of waves)
-    const int per = (256 / KOR_GROUPS + cells - 1) / cells;
-    return KOR_GROUPS * (per < 1 ? 1 : per);
+    (void)cells;
+    return KOR_GROUPS * h->kntile;
 }
 
-int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Qpart, double *mpart, int cells)
+int launch_kor_gather(gdt_context *h, const double *V, double *Lpart, double *Qpart, double *mpart, int cells)
 {
     const int B = kor_gather_blocks(h, cells);
-    const int dps = (h->n_pnt + B / KOR_GROUPS - 1) / (B / KOR_GROUPS);
     KernelTimer kt(h, "kor_gather");
-    hipLaunchKernelGGL(k_kor_gather, dim3(B, cells), dim3(KOR_THREADS), 0, h->stream, h->n_pnt, h->nn, dps,
-                       h->kgptr.as<int32_t>(), h->kgent.as<uint32_t>(), h->kgcoef.as<double>(), (const double2 *)DW,
-                       h->xi.as<double>(), Lpart, Qpart, mpart);
+    hipLaunchKernelGGL(k_kor_gather, dim3(B, cells), dim3(TILE_THREADS), sizeof(double2) * h->kcap, h->stream,
+                       h->n_pnt, h->nn, h->kntile, h->kcap, h->ktile.as<int4>(), h->kgptr.as<int32_t>(),
+                       h->kgent.as<uint2>(), (const double2 *)V, h->xi.as<double>(), Lpart, Qpart, mpart);
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
@@ -494,14 +562,13 @@ int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const doub
                      const double *Lpart, const double *Qpart, double *Qout, double *Lout, double *nu_out, double *X,
                      int cells, double dt, double *f_upd)
 {
+    GDT_CHECK(G == 1 || G == KOR_GROUPS, GDT_E_ARG, "kor_apply: %d partials per node", G);
     const dim3 grid((h->n_pnt + 255) / 256, cells);
     KernelTimer kt(h, "kor_apply");
-    if (correct)
-        hipLaunchKernelGGL(k_kor_apply<true>, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, G, mpart,
-                           h->xi.as<double>(), f, Lpart, Qpart, Qout, Lout, nu_out, X, dt, f_upd);
-    else
-        hipLaunchKernelGGL(k_kor_apply<false>, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, G, mpart,
-                           h->xi.as<double>(), f, Lpart, Qpart, Qout, Lout, nu_out, X, dt, f_upd);
+    auto kern = correct ? (G == 1 ? k_kor_apply<true, 1> : k_kor_apply<true, KOR_GROUPS>)
+                        : (G == 1 ? k_kor_apply<false, 1> : k_kor_apply<false, KOR_GROUPS>);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, mpart, h->xi.as<double>(), f, Lpart,
+                       Qpart, Qout, Lout, nu_out, X, dt, f_upd);
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
