@@ -60,6 +60,20 @@ def entropy(f, d3v):
     return float(np.sum(fp * np.log(fp)) * d3v)
 
 
+def profiled_traffic():
+    """HBM bytes per launch per kernel, measured with rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) and
+    recorded in profiles/; empty if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "korobov_step_32cube_2e21.md")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic":
+                out[parts[1]] = int(parts[2])
+    return out
+
+
 def cpu_baseline(xi, f0, n_pnt, d3v, budget_s=15.0):
     """FP64 oracle (C, OpenMP over lattice nodes) on a bounded number of
     evaluations of the same workload: stencil once (untimed), then corrected
@@ -144,6 +158,7 @@ def main():
         # alternative fused kernel: 20-byte entry records + f + ptr + centres
         "kor_fused": ne * 20 + 4 * (n + 1) + 8 * n + 24 * n + 16 * n + 256 * 160,
     }
+    traffic = profiled_traffic()
     roof = None
     kstats = {}
     if not args.no_roofline:
@@ -159,7 +174,9 @@ def main():
         dom = max(kstats, key=lambda k: kstats[k]["avg_us"])
         ach = kstats[dom]["gbs"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic.get(dom),
+                "traffic_source": "profiles/korobov_step_32cube_2e21.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                if dom in traffic else None,
                 "avg_us": round(kstats[dom]["avg_us"], 3), "alg_bytes": alg[dom],
                 "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                             for k, v in kstats.items()}}
