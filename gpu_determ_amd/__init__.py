@@ -35,6 +35,8 @@ ABI_SYMBOLS = (
     "gdt_collide_matrix", "gdt_collide_korobov", "gdt_correct", "gdt_evaluate", "gdt_relax", "gdt_run_steps",
     "gdt_kernel_stats", "gdt_kernel_stats_reset", "gdt_transpose", "gdt_device_synchronize",
     "gdt_convert_f32_to_f64", "gdt_convert_f64_to_f32",
+    "gdt_batch_create", "gdt_batch_destroy", "gdt_batch_set_output", "gdt_batch_config", "gdt_batch_tune",
+    "gdt_batch_run", "gdt_batch_synchronize", "gdt_batch_info",
 )
 
 _lib = None
@@ -95,6 +97,14 @@ def lib():
         "gdt_kernel_stats": [vp, C.c_char_p, dp, C.POINTER(C.c_long)],
         "gdt_kernel_stats_reset": [vp],
         "gdt_transpose": [vp, vp, vp, C.c_int, C.c_int],
+        "gdt_batch_create": [C.POINTER(vp), C.c_int, C.POINTER(vp), C.POINTER(vp), ip],
+        "gdt_batch_destroy": [vp],
+        "gdt_batch_set_output": [vp, C.POINTER(vp)],
+        "gdt_batch_config": [vp, C.c_int, C.c_int],
+        "gdt_batch_tune": [vp, C.c_uint, ip, ip, dp],
+        "gdt_batch_run": [vp, C.c_double, C.c_int, C.c_uint],
+        "gdt_batch_synchronize": [vp],
+        "gdt_batch_info": [vp, ip, ip, ip],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -285,6 +295,56 @@ class Context:
 
     def transpose(self, src, dst, rows, cols):
         check(lib().gdt_transpose(self.h, src.ptr, dst.ptr, int(rows), int(cols)))
+
+
+class Batch:
+    """Uneven batch of cells over several Korobov contexts (gdt_batch_*).
+
+    classes: Contexts with a stencil; fs: DeviceArrays [cells][n_pnt] per
+    class (kept referenced: the batch holds their device pointers)."""
+
+    def __init__(self, classes, fs, Qs=None):
+        self.classes, self.fs, self.Qs = list(classes), list(fs), Qs
+        n = len(self.classes)
+        hs = (C.c_void_p * n)(*[c.h for c in self.classes])
+        fp = (C.c_void_p * n)(*[f.ptr for f in self.fs])
+        cells = (C.c_int * n)(*[f.shape[0] for f in self.fs])
+        h = C.c_void_p()
+        check(lib().gdt_batch_create(C.byref(h), n, hs, fp, cells))
+        self.h = h
+        if Qs is not None:
+            qp = (C.c_void_p * n)(*[q.ptr for q in Qs])
+            check(lib().gdt_batch_set_output(self.h, qp))
+
+    def config(self, seg_cells, node_q):
+        check(lib().gdt_batch_config(self.h, int(seg_cells), int(node_q)))
+
+    def tune(self, flags=FLAG_CORRECT):
+        k, q, ms = C.c_int(), C.c_int(), C.c_double()
+        check(lib().gdt_batch_tune(self.h, flags, C.byref(k), C.byref(q), C.byref(ms)))
+        return k.value, q.value, ms.value
+
+    def run(self, dt, steps, flags=FLAG_CORRECT):
+        check(lib().gdt_batch_run(self.h, float(dt), int(steps), flags))
+
+    def sync(self):
+        check(lib().gdt_batch_synchronize(self.h))
+
+    def info(self):
+        k, q, ns = C.c_int(), C.c_int(), C.c_int()
+        check(lib().gdt_batch_info(self.h, C.byref(k), C.byref(q), C.byref(ns)))
+        return {"seg_cells": k.value, "node_q": q.value, "segments": ns.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gdt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def korobov_pick(M, seed, ncand=8):

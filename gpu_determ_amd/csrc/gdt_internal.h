@@ -114,6 +114,35 @@ int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells);
 int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells);
 int launch_cmp_to_box(gdt_context *h, const double *f_cmp, double *f_box, int cells, int layout);
 int launch_box_to_cmp(gdt_context *h, const double *f_box, double *f_cmp, int cells, int layout);
+// ---- uneven batches (batch_abi.hip, korobov_kernels.hip) -------------------
+// One velocity grid (class) of a batch: its stencil, the caller's f / Q of
+// all its cells, and the scratch of the current segment's cells (slots).
+struct KorClassDev {
+    int npts, ntile, cap, nparts;  // nparts = KOR_NGROUPS * ntile
+    long nn;
+    double K;
+    const uint4 *nrec;
+    const uint32_t *nd2;
+    const int4 *tiles;
+    const int32_t *gptr;
+    const uint2 *ent;
+    const double *xi;
+    double *f;     // [cells][npts], updated in place by steps
+    double *Qout;  // [cells][npts] or NULL
+    double2 *V;    // [slots][nn]
+    double *Lp;    // [slots][G][npts]
+    double *Qp;    // [slots][G][npts]
+    double *mp;    // [slots][20][nparts]
+    int node_blocks;  // node work items per cell
+    int pad;
+};
+// Work item: (class, cell, block within the cell, slot)
+int launch_batch_node(hipStream_t s, const KorClassDev *cls, const int4 *items, int nitems, int blocks, int lds);
+int launch_batch_gather(hipStream_t s, const KorClassDev *cls, const int4 *items, int nitems, int blocks, int lds);
+int launch_batch_apply(hipStream_t s, bool correct, const KorClassDev *cls, const int4 *items, int nitems, int blocks,
+                       double dt);
+int batch_set_lds(int lds_node, int lds_gather);
+
 // ---- Korobov kernels (korobov_kernels.hip) --------------------------------
 int kor_blocks(gdt_context *h, int cells);
 int kor_setup(gdt_context *h, int cells);

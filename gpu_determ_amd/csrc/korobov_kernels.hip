@@ -92,15 +92,15 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts)
 // dependency chains (LDS reads, log / exp) overlap; their records are loaded
 // before the staging of f, so that latency overlaps too.  Further nodes (more
 // than two per thread) are handled one at a time.
-__global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, const uint4 *__restrict__ nrec,
-                                                           const uint32_t *__restrict__ nd2, double K,
-                                                           const double *__restrict__ f, double2 *__restrict__ V)
+// Block b of a grid of B blocks per cell: nodes b*KOR_THREADS + tid (+ k B
+// KOR_THREADS) of the cell whose f is fc; values to out (the cell's V).
+__device__ __forceinline__ void node_block(double *fs, int npts, long nn, const uint4 *__restrict__ nrec,
+                                           const uint32_t *__restrict__ nd2, double K, const double *__restrict__ fc,
+                                           double2 *__restrict__ out, int b, int B)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double fs[];
-    const int c = blockIdx.y;
-    const long stride = (long)gridDim.x * KOR_THREADS;
-    const long ta = (long)blockIdx.x * KOR_THREADS + threadIdx.x, tb = ta + stride;
+    const long stride = (long)B * KOR_THREADS;
+    const long ta = (long)b * KOR_THREADS + threadIdx.x, tb = ta + stride;
     uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
     uint32_t ya = 0u, yb = 0u;
     if (ta < nn) {
@@ -111,9 +111,8 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, con
         xb = nrec[tb];
         yb = nd2[tb];
     }
-    stage_f(fs, f + (size_t)c * npts, npts);
+    stage_f(fs, fc, npts);
     __syncthreads();
-    double2 *out = V + (size_t)c * nn;
     double Da, Wa, ra, Db, Wb, rb;
     node_terms(xa, ya, fs, K, Da, Wa, ra);
     node_terms(xb, yb, fs, K, Db, Wb, rb);
@@ -126,13 +125,22 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, con
     }
 }
 
+__global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, const uint4 *__restrict__ nrec,
+                                                           const uint32_t *__restrict__ nd2, double K,
+                                                           const double *__restrict__ f, double2 *__restrict__ V)
+{
+    extern __shared__ double fs[];
+    const int c = blockIdx.y;
+    node_block(fs, npts, nn, nrec, nd2, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // block helpers
 // ---------------------------------------------------------------------------
 // Block sum of the 20 moment sums (DPP row sums, then the rows in a fixed
-// order), written to mpart[cell][moment][block].
+// order), written to mp[moment * nparts + b] (mp: the cell's moment sums).
 template <int NT>
-__device__ __forceinline__ void block_moments(double (&m)[20], double (*rpart)[20], double *mpart, int c)
+__device__ __forceinline__ void block_moments(double (&m)[20], double (*rpart)[20], double *mp, int b, int nparts)
 {
 #pragma unroll
     for (int i = 0; i < 20; i++) m[i] += dpp_f64<DPP_QUAD_XOR1>(m[i]);
@@ -149,7 +157,7 @@ __device__ __forceinline__ void block_moments(double (&m)[20], double (*rpart)[2
     if (threadIdx.x < 20) {
         double sm = 0.0;
         for (int r = 0; r < NT / 16; r++) sm += rpart[r][threadIdx.x];
-        mpart[((size_t)c * 20 + threadIdx.x) * gridDim.x + blockIdx.x] = sm;
+        mp[(size_t)threadIdx.x * nparts + b] = sm;
     }
 }
 
@@ -188,20 +196,16 @@ __device__ __forceinline__ void zero_empty(int ga, int gb, const int32_t *ptr, d
 #define TILE_DPT (KOR_TILE_DST / TILE_THREADS)  // destinations per thread
 #define TILE_BATCH 20
 
-__global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, int ntile, int cap,
-                                                              const int4 *__restrict__ tiles,
-                                                              const int32_t *__restrict__ gptr,
-                                                              const uint2 *__restrict__ ent,
-                                                              const double2 *__restrict__ V,
-                                                              const double *__restrict__ xi,
-                                                              double *__restrict__ Lpart, double *__restrict__ Qpart,
-                                                              double *__restrict__ mpart)
+// Block b (group b % G, tile b / G) of a cell: Vc its node values, Lp / Qp its
+// group partials [G][npts], mp its moment sums [20][nparts].
+__device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], int npts, int ntile, int cap,
+                                             const int4 *__restrict__ tiles, const int32_t *__restrict__ gptr,
+                                             const uint2 *__restrict__ ent, const double2 *__restrict__ Vc,
+                                             const double *__restrict__ xi, double *__restrict__ Lp,
+                                             double *__restrict__ Qp, double *__restrict__ mp, int b, int nparts)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double2 ql[];
-    __shared__ double rpart[TILE_WAVES * 4][20];
-    const int c = blockIdx.y;
-    const int g = blockIdx.x % KOR_GROUPS, k = blockIdx.x / KOR_GROUPS;
+    const int g = b % KOR_GROUPS, k = b / KOR_GROUPS;
     const int4 tl = tiles[(size_t)g * ntile + k];
     const int d0 = tl.x, d1 = tl.y, e0 = tl.z, e1 = tl.w;
     const int32_t *ptr = gptr + (size_t)g * (npts + 1);
@@ -220,7 +224,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, 
             pz[i] = xi[3 * d + 2];
         }
     }
-    const double2 *Vc = V + (size_t)c * nn;
     for (int w0 = e0; w0 < e1; w0 += cap) {
         const int w1 = min(e1, w0 + cap);
         if (w0 > e0) __syncthreads();  // the previous pass's reads are done
@@ -252,8 +255,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, 
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < TILE_DPT; i++) {
-            const int a = max(ea[i], w0), b = min(eb[i], w1);
-            for (int e = a; e < b; e++) {
+            const int a = max(ea[i], w0), bb = min(eb[i], w1);
+            for (int e = a; e < bb; e++) {
                 const double2 t = ql[e - w0];
                 q[i] += t.x;
                 l[i] += t.y;
@@ -267,13 +270,30 @@ __global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, 
     for (int i = 0; i < TILE_DPT; i++) {
         const int d = d0 + threadIdx.x + i * TILE_THREADS;
         if (d < d1) {
-            const size_t po = ((size_t)c * KOR_GROUPS + g) * npts + d;
-            Qpart[po] = q[i];
-            Lpart[po] = l[i];
+            const size_t po = (size_t)g * npts + d;
+            Qp[po] = q[i];
+            Lp[po] = l[i];
             moments_add_psi(m, px[i], py[i], pz[i], q[i], l[i]);
         }
     }
-    block_moments<TILE_THREADS>(m, rpart, mpart, c);
+    block_moments<TILE_THREADS>(m, rpart, mp, b, nparts);
+}
+
+__global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, int ntile, int cap,
+                                                              const int4 *__restrict__ tiles,
+                                                              const int32_t *__restrict__ gptr,
+                                                              const uint2 *__restrict__ ent,
+                                                              const double2 *__restrict__ V,
+                                                              const double *__restrict__ xi,
+                                                              double *__restrict__ Lpart, double *__restrict__ Qpart,
+                                                              double *__restrict__ mpart)
+{
+    extern __shared__ double2 ql[];
+    __shared__ double rpart[TILE_THREADS / 16][20];
+    const int c = blockIdx.y;
+    const size_t po = (size_t)c * KOR_GROUPS * npts;
+    gather_block(ql, rpart, npts, ntile, cap, tiles, gptr, ent, V + (size_t)c * nn, xi, Lpart + po, Qpart + po,
+                 mpart + (size_t)c * 20 * gridDim.x, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -374,7 +394,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
             cl = readlane_f64(l, WAVE - 1);
         }
     }
-    block_moments<KOR_THREADS>(m, rpart, mpart, c);
+    block_moments<KOR_THREADS>(m, rpart, mpart + (size_t)c * 20 * gridDim.x, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -385,31 +405,29 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // partials in group order and writes
 //   Q = Q0 - L X.psi,  loss = L,  nu = (L/f)(1 + X.psi),  f <- f + dt Q.
 #define APPLY_ROUNDS 2  // block sums per thread and moment with loads in flight (512 blocks: one round)
+#define APPLY_THREADS 256
+// Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
+// mp [20][nparts], fc [npts], Lp / Qp [G][npts], outputs [npts], X [5].
 template <bool CORR, int G>
-__global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, const double *__restrict__ mpart,
-                                                   const double *__restrict__ xi, const double *f,
-                                                   const double *__restrict__ Lpart,
-                                                   const double *__restrict__ Qpart, double *__restrict__ Qout,
-                                                   double *__restrict__ Lout, double *__restrict__ nu_out,
-                                                   double *__restrict__ Xout, double dt, double *f_upd)
+__device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int npts, int nparts,
+                                            const double *__restrict__ mp, const double *__restrict__ xi,
+                                            const double *fc, const double *__restrict__ Lp,
+                                            const double *__restrict__ Qp, double *__restrict__ Qout,
+                                            double *__restrict__ Lout, double *__restrict__ nu_out,
+                                            double *__restrict__ Xout, double dt, double *f_upd, int b)
 {
-    __shared__ double red[16][20];
-    __shared__ double Xs[5];
-    const int c = blockIdx.y;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t o = (size_t)c * npts + v;
+    const int v = b * APPLY_THREADS + threadIdx.x;
     double x = 0.0, y = 0.0, z = 0.0, L = 0.0, fv = 0.0, q0 = 0.0;
     if (v < npts) {
         x = xi[3 * v];
         y = xi[3 * v + 1];
         z = xi[3 * v + 2];
-        fv = f[o];
+        fv = fc[v];
         double qg[G], lg[G];
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            const size_t po = ((size_t)c * G + g) * npts + v;
-            qg[g] = Qpart[po];
-            lg[g] = Lpart[po];
+            qg[g] = Qp[(size_t)g * npts + v];
+            lg[g] = Lp[(size_t)g * npts + v];
         }
 #pragma unroll
         for (int g = 0; g < G; g++) {
@@ -424,13 +442,13 @@ __global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, const d
         double m[20];
 #pragma unroll
         for (int a = 0; a < 20; a++) m[a] = 0.0;
-        for (int p0 = 0; p0 < nparts; p0 += APPLY_ROUNDS * 256) {
+        for (int p0 = 0; p0 < nparts; p0 += APPLY_ROUNDS * APPLY_THREADS) {
             double vv[APPLY_ROUNDS][20];
 #pragma unroll
             for (int k = 0; k < APPLY_ROUNDS; k++) {
-                const int p = p0 + k * 256 + threadIdx.x;
+                const int p = p0 + k * APPLY_THREADS + threadIdx.x;
 #pragma unroll
-                for (int a = 0; a < 20; a++) vv[k][a] = p < nparts ? mpart[((size_t)c * 20 + a) * nparts + p] : 0.0;
+                for (int a = 0; a < 20; a++) vv[k][a] = p < nparts ? mp[(size_t)a * nparts + p] : 0.0;
             }
 #pragma unroll
             for (int k = 0; k < APPLY_ROUNDS; k++)
@@ -453,7 +471,7 @@ __global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, const d
             double sm[20], X[5];
             for (int a = 0; a < 20; a++) {
                 double t = 0.0;
-                for (int r = 0; r < 16; r++) t += red[r][a];
+                for (int r = 0; r < APPLY_THREADS / 16; r++) t += red[r][a];
                 sm[a] = t;
             }
             moments_solve(sm, X);
@@ -463,15 +481,120 @@ __global__ __launch_bounds__(256) void k_kor_apply(int npts, int nparts, const d
         Xs[threadIdx.x] = 0.0;
     }
     __syncthreads();
-    if (Xout && blockIdx.x == 0 && threadIdx.x < 5) Xout[5 * c + threadIdx.x] = Xs[threadIdx.x];
-    if (v >= npts) return;
-    const double e = x * x + y * y + z * z;
-    const double s = Xs[0] + Xs[1] * x + Xs[2] * y + Xs[3] * z + Xs[4] * e;
-    const double q = q0 - L * s;
-    if (Qout) Qout[o] = q;
-    if (Lout) Lout[o] = L;
-    if (nu_out) nu_out[o] = fv > 0.0 ? (L / fv) * (1.0 + s) : 0.0;
-    if (dt != 0.0) f_upd[o] = fv + dt * q;
+    if (Xout && b == 0 && threadIdx.x < 5) Xout[threadIdx.x] = Xs[threadIdx.x];
+    if (v < npts) {  // no early return: persistent callers run further items after this one
+        const double e = x * x + y * y + z * z;
+        const double s = Xs[0] + Xs[1] * x + Xs[2] * y + Xs[3] * z + Xs[4] * e;
+        const double q = q0 - L * s;
+        if (Qout) Qout[v] = q;
+        if (Lout) Lout[v] = L;
+        if (nu_out) nu_out[v] = fv > 0.0 ? (L / fv) * (1.0 + s) : 0.0;
+        if (dt != 0.0) f_upd[v] = fv + dt * q;
+    }
+}
+
+template <bool CORR, int G>
+__global__ __launch_bounds__(APPLY_THREADS) void k_kor_apply(int npts, int nparts, const double *__restrict__ mpart,
+                                                             const double *__restrict__ xi, const double *f,
+                                                             const double *__restrict__ Lpart,
+                                                             const double *__restrict__ Qpart,
+                                                             double *__restrict__ Qout, double *__restrict__ Lout,
+                                                             double *__restrict__ nu_out, double *__restrict__ Xout,
+                                                             double dt, double *f_upd)
+{
+    __shared__ double red[APPLY_THREADS / 16][20];
+    __shared__ double Xs[5];
+    const int c = blockIdx.y;
+    const size_t o = (size_t)c * npts, po = (size_t)c * G * npts;
+    apply_block<CORR, G>(red, Xs, npts, nparts, mpart + (size_t)c * 20 * nparts, xi, f + o, Lpart + po, Qpart + po,
+                         Qout ? Qout + o : nullptr, Lout ? Lout + o : nullptr, nu_out ? nu_out + o : nullptr,
+                         Xout ? Xout + 5 * c : nullptr, dt, f_upd ? f_upd + o : nullptr, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// uneven batches: persistent work-list kernels
+// ---------------------------------------------------------------------------
+// Each kernel walks a static list of (class, cell, block, slot) items; block p
+// takes items p, p + gridDim.x, ...  Every item runs exactly the body one
+// block of the single-cell kernel runs, so per-cell results are bitwise those
+// of a non-batched run.  Gather lists keep item i in group i % G, and the
+// grid is a multiple of G, so each group stays on one XCD.  No atomics and no
+// waits between blocks: every block finishes its own items and exits.
+__global__ __launch_bounds__(KOR_THREADS) void k_batch_node(const KorClassDev *__restrict__ cls,
+                                                             const int4 *__restrict__ items, int nitems)
+{
+    extern __shared__ double fs[];
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        if (i != (int)blockIdx.x) __syncthreads();  // the previous item's LDS reads are done
+        const int4 it = items[i];
+        const KorClassDev C = cls[it.x];
+        node_block(fs, C.npts, C.nn, C.nrec, C.nd2, C.K, C.f + (size_t)it.y * C.npts, C.V + (size_t)it.w * C.nn,
+                   it.z, C.node_blocks);
+    }
+}
+
+__global__ __launch_bounds__(TILE_THREADS) void k_batch_gather(const KorClassDev *__restrict__ cls,
+                                                                const int4 *__restrict__ items, int nitems)
+{
+    extern __shared__ double2 ql[];
+    __shared__ double rpart[TILE_THREADS / 16][20];
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        if (i != (int)blockIdx.x) __syncthreads();
+        const int4 it = items[i];
+        const KorClassDev C = cls[it.x];
+        const size_t po = (size_t)it.w * KOR_GROUPS * C.npts;
+        gather_block(ql, rpart, C.npts, C.ntile, C.cap, C.tiles, C.gptr, C.ent, C.V + (size_t)it.w * C.nn, C.xi,
+                     C.Lp + po, C.Qp + po, C.mp + (size_t)it.w * 20 * C.nparts, it.z, C.nparts);
+    }
+}
+
+template <bool CORR>
+__global__ __launch_bounds__(APPLY_THREADS) void k_batch_apply(const KorClassDev *__restrict__ cls,
+                                                                const int4 *__restrict__ items, int nitems, double dt)
+{
+    __shared__ double red[APPLY_THREADS / 16][20];
+    __shared__ double Xs[5];
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        if (i != (int)blockIdx.x) __syncthreads();
+        const int4 it = items[i];
+        const KorClassDev C = cls[it.x];
+        const size_t o = (size_t)it.y * C.npts, po = (size_t)it.w * KOR_GROUPS * C.npts;
+        apply_block<CORR, KOR_GROUPS>(red, Xs, C.npts, C.nparts, C.mp + (size_t)it.w * 20 * C.nparts, C.xi, C.f + o,
+                                      C.Lp + po, C.Qp + po, C.Qout ? C.Qout + o : nullptr, nullptr, nullptr, nullptr,
+                                      dt, dt != 0.0 ? C.f + o : nullptr, it.z);
+    }
+}
+
+int batch_set_lds(int lds_node, int lds_gather)
+{
+    GDT_HIP(hipFuncSetAttribute((const void *)k_batch_node, hipFuncAttributeMaxDynamicSharedMemorySize, lds_node));
+    GDT_HIP(hipFuncSetAttribute((const void *)k_batch_gather, hipFuncAttributeMaxDynamicSharedMemorySize, lds_gather));
+    return GDT_OK;
+}
+
+int launch_batch_node(hipStream_t s, const KorClassDev *cls, const int4 *items, int nitems, int blocks, int lds)
+{
+    hipLaunchKernelGGL(k_batch_node, dim3(blocks), dim3(KOR_THREADS), lds, s, cls, items, nitems);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+int launch_batch_gather(hipStream_t s, const KorClassDev *cls, const int4 *items, int nitems, int blocks, int lds)
+{
+    hipLaunchKernelGGL(k_batch_gather, dim3(blocks), dim3(TILE_THREADS), lds, s, cls, items, nitems);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+int launch_batch_apply(hipStream_t s, bool correct, const KorClassDev *cls, const int4 *items, int nitems, int blocks,
+                       double dt)
+{
+    if (correct)
+        hipLaunchKernelGGL(k_batch_apply<true>, dim3(blocks), dim3(APPLY_THREADS), 0, s, cls, items, nitems, dt);
+    else
+        hipLaunchKernelGGL(k_batch_apply<false>, dim3(blocks), dim3(APPLY_THREADS), 0, s, cls, items, nitems, dt);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -157,6 +157,30 @@ int gdt_relax(gdt_handle h, double *f, const double *nu, const double *gain, int
  * GDT_FLAG_GRAPH the step is captured once as a hipGraph and replayed. */
 int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int steps, unsigned flags);
 
+/* ---- uneven batches (no reference counterpart: cells of several grids) ---
+ * Classes are handles with a Korobov stencil built; f[s] is device FP64
+ * [cells[s]][n_pnt of class s] (cell-major), updated in place by steps.
+ * Each run step evaluates every cell (corrected if GDT_FLAG_CORRECT) and,
+ * for dt != 0, applies f += dt Q: per segment of cells, three persistent
+ * work-list kernels; with GDT_FLAG_GRAPH the whole step is one hipGraph.
+ * Every cell's result is bitwise that of gdt_run_steps on the cell alone.
+ * The batch keeps the f / Q pointers and the class handles: they must
+ * outlive it. */
+typedef struct gdt_batch *gdt_batch_handle;
+int gdt_batch_create(gdt_batch_handle *b, int nclass, const gdt_handle *classes, double *const *f,
+                     const int *cells);
+int gdt_batch_destroy(gdt_batch_handle b);
+/* Q of the last run step per class ([cells][n_pnt], device), or NULL. */
+int gdt_batch_set_output(gdt_batch_handle b, double *const *Q);
+/* Explicit launch shape: cells per segment, nodes per thread of the node kernel (>= 2). */
+int gdt_batch_config(gdt_batch_handle b, int seg_cells, int node_q);
+/* Autotuner: times every (segment size, nodes per thread) candidate on the
+ * batch's cells (evaluation only, f unchanged) and keeps the fastest. */
+int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node_q, double *best_ms);
+int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags);
+int gdt_batch_synchronize(gdt_batch_handle b);
+int gdt_batch_info(gdt_batch_handle b, int *seg_cells, int *node_q, int *segments);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Kernel timing under GDT_FLAG_TIME: total milliseconds and launch count of
  * the kernel `name` since the last reset. */

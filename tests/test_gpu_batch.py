@@ -1,0 +1,103 @@
+"""Uneven batches (gdt_batch_*): cells of several velocity grids in persistent
+work-list launches.  The contract is per-cell equality with non-batched runs,
+bitwise, whatever the segment size, node blocking, tuning or graph replay."""
+import numpy as np
+import pytest
+
+import gpu_determ_amd as G
+
+from helpers import mixture
+
+pytestmark = pytest.mark.gpu
+
+VMAX = 4.8
+DT = 0.01
+
+
+def make_classes(sizes, M=1 << 16, seed=5):
+    out = []
+    for n in sizes:
+        ctx = G.Context((n,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+        ctx.korobov_build(M, seed=seed, s0=0.25)
+        out.append(ctx)
+    return out
+
+
+def cell_fs(ctx, cells, rng):
+    return np.stack([mixture(ctx.xi, rng, comps=1 + (c % 3)) for c in range(cells)])
+
+
+def single_runs(ctx, f0, steps, flags):
+    """Each cell alone through gdt_run_steps (cells = 1)."""
+    out = []
+    for c in range(f0.shape[0]):
+        d = G.DeviceArray.from_host(f0[c:c + 1])
+        ctx.run_steps(G.METHOD_KOROBOV, d, 1, DT, steps, flags)
+        ctx.sync()
+        out.append(d.get()[0])
+    return np.stack(out)
+
+
+@pytest.fixture(scope="module")
+def classes():
+    return make_classes((12, 16, 20))
+
+
+@pytest.mark.parametrize("seg,q", [(64, 2), (2, 4), (1, 8)])
+def test_batch_steps_bitwise_equal_single(classes, seg, q):
+    print("batch steps: seg %d q %d" % (seg, q), flush=True)
+    rng = np.random.default_rng(11)
+    cells = [3, 2, 2]
+    f0 = [cell_fs(c, k, rng) for c, k in zip(classes, cells)]
+    fd = [G.DeviceArray.from_host(x) for x in f0]
+    b = G.Batch(classes, fd)
+    b.config(seg, q)
+    assert b.info()["segments"] == -(-sum(cells) // seg)
+    flags = G.FLAG_CORRECT | G.FLAG_GRAPH
+    b.run(DT, 2, flags)
+    b.sync()
+    for ctx, x0, d in zip(classes, f0, fd):
+        ref = single_runs(ctx, x0, 2, G.FLAG_CORRECT)
+        got = d.get()
+        assert np.array_equal(got, ref), np.max(np.abs(got - ref))
+    b.close()
+
+
+def test_batch_tune_keeps_f_and_q_output(classes):
+    print("batch tune + Q output", flush=True)
+    rng = np.random.default_rng(12)
+    cells = [2, 3, 1]
+    f0 = [cell_fs(c, k, rng) for c, k in zip(classes, cells)]
+    fd = [G.DeviceArray.from_host(x) for x in f0]
+    qd = [G.DeviceArray((k, c.n_pnt)) for c, k in zip(classes, cells)]
+    b = G.Batch(classes, fd, qd)
+    seg, q, ms = b.tune(G.FLAG_CORRECT)
+    assert seg >= 1 and q in (2, 8, 32, 128) and ms > 0.0
+    for x0, d in zip(f0, fd):  # tuning evaluates only
+        assert np.array_equal(d.get(), x0)
+    b.run(0.0, 1, G.FLAG_CORRECT)  # evaluation: Q out, f unchanged
+    b.sync()
+    for ctx, x0, d, qq in zip(classes, f0, fd, qd):
+        assert np.array_equal(d.get(), x0)
+        _, _, Qref = ctx.evaluate(G.METHOD_KOROBOV, G.DeviceArray.from_host(x0), x0.shape[0], G.FLAG_CORRECT)
+        ctx.sync()
+        assert np.array_equal(qq.get(), Qref.get())
+    b.close()
+
+
+def test_batch_eager_equals_graph(classes):
+    print("batch eager vs graph", flush=True)
+    rng = np.random.default_rng(13)
+    cells = [2, 2, 2]
+    f0 = [cell_fs(c, k, rng) for c, k in zip(classes, cells)]
+    outs = []
+    for flags in (G.FLAG_CORRECT, G.FLAG_CORRECT | G.FLAG_GRAPH):
+        fd = [G.DeviceArray.from_host(x) for x in f0]
+        b = G.Batch(classes, fd)
+        b.config(4, 2)
+        b.run(DT, 3, flags)
+        b.sync()
+        outs.append([d.get() for d in fd])
+        b.close()
+    for a, c in zip(*outs):
+        assert np.array_equal(a, c)
