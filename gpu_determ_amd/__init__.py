@@ -26,6 +26,10 @@ FLAG_TIME = 4
 FLAG_COURANT = 8
 FLAG_FUSED = 16
 
+GEMM_BF16 = 0
+GEMM_BF16_SPLIT = 1
+GEMM_F32 = 2
+
 # every symbol that include/gdt.h declares
 ABI_SYMBOLS = (
     "gdt_last_error", "gdt_version", "gdt_device_count", "gdt_malloc", "gdt_free", "gdt_memcpy_htod",
@@ -37,6 +41,7 @@ ABI_SYMBOLS = (
     "gdt_convert_f32_to_f64", "gdt_convert_f64_to_f32",
     "gdt_batch_create", "gdt_batch_destroy", "gdt_batch_set_output", "gdt_batch_config", "gdt_batch_tune",
     "gdt_batch_run", "gdt_batch_synchronize", "gdt_batch_info",
+    "gdt_gemm_nt", "gdt_split_f64", "gdt_matrix_mfma",
 )
 
 _lib = None
@@ -105,6 +110,9 @@ def lib():
         "gdt_batch_run": [vp, C.c_double, C.c_int, C.c_uint],
         "gdt_batch_synchronize": [vp],
         "gdt_batch_info": [vp, ip, ip, ip],
+        "gdt_gemm_nt": [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int],
+        "gdt_split_f64": [vp, C.c_long, vp, vp, vp],
+        "gdt_matrix_mfma": [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_uint],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -285,6 +293,15 @@ class Context:
     def run_steps(self, method, f, cells, dt, steps, flags=FLAG_CORRECT):
         check(lib().gdt_run_steps(self.h, int(method), f.ptr, int(cells), float(dt), int(steps), int(flags)))
 
+    def matrix_mfma(self, mode, f, cells, flags=FLAG_CORRECT):
+        """Batched analytic-matrix operator on the matrix cores (16^3 box):
+        returns (nu corrected, gain, Q) DeviceArrays [cells][4096]."""
+        nu = DeviceArray((cells, self.n_pnt))
+        gain = DeviceArray((cells, self.n_pnt))
+        Q = DeviceArray((cells, self.n_pnt))
+        check(lib().gdt_matrix_mfma(self.h, int(mode), f.ptr, nu.ptr, gain.ptr, Q.ptr, int(cells), int(flags)))
+        return nu, gain, Q
+
     def kernel_stats(self, name):
         ms, cnt = C.c_double(), C.c_long()
         check(lib().gdt_kernel_stats(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
@@ -345,6 +362,31 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def split_f64(x, hi=True, lo=True, f32=True):
+    """FP64 DeviceArray -> (bf16 hi, bf16 lo, fp32) DeviceArrays (None where not asked)."""
+    n = int(np.prod(x.shape))
+    h = DeviceArray(x.shape, np.uint16) if hi else None
+    l_ = DeviceArray(x.shape, np.uint16) if lo else None
+    f = DeviceArray(x.shape, np.float32) if f32 else None
+    check(lib().gdt_split_f64(x.ptr, n, h.ptr if h else None, l_.ptr if l_ else None, f.ptr if f else None))
+    return h, l_, f
+
+
+def gemm_nt(mode, A, B, accumulate=None):
+    """C = A B^T on the matrix cores (FP32 result).  A: [M][K], B: [N][K] FP64
+    DeviceArrays, converted per mode; returns the FP32 DeviceArray C [M][N]."""
+    M, K = A.shape
+    N = B.shape[0]
+    ah, al, af = split_f64(A)
+    bh, bl, bf = split_f64(B)
+    C_ = accumulate if accumulate is not None else DeviceArray((M, N), np.float32)
+    if mode == GEMM_F32:
+        check(lib().gdt_gemm_nt(mode, M, N, K, af.ptr, None, bf.ptr, None, C_.ptr, N, int(accumulate is not None)))
+    else:
+        check(lib().gdt_gemm_nt(mode, M, N, K, ah.ptr, al.ptr, bh.ptr, bl.ptr, C_.ptr, N, int(accumulate is not None)))
+    return C_
 
 
 def korobov_pick(M, seed, ncand=8):

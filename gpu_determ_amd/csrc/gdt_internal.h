@@ -79,6 +79,11 @@ struct gdt_context {
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
     int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
 
+    // batched matrix form on the matrix cores (16^3 box): operands built from
+    // the tables (B = Toeplitz b, AwT = a by parity class; bf16 hi / lo, fp32)
+    DevBuf mx_bh, mx_bl, mx_bf, mx_ah, mx_al, mx_af, s_mx;
+    bool mx_ready = false;
+
     // scratch
     DevBuf s_fbox, s_out1, s_out2, s_vals, s_loss, s_Q, s_nu, s_tmp, s_dt, s_mpart, s_part;
 
@@ -114,6 +119,17 @@ int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells);
 int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells);
 int launch_cmp_to_box(gdt_context *h, const double *f_cmp, double *f_box, int cells, int layout);
 int launch_box_to_cmp(gdt_context *h, const double *f_box, double *f_cmp, int cells, int layout);
+// ---- matrix cores (mfma_gemm.hip, matrix_mfma.hip) ---------------------------
+int launch_gemm_nt(hipStream_t s, int mode, int M, int N, int K, const void *Ah, const void *Al, const void *Bh,
+                   const void *Bl, float *C, int ldc, int accumulate);
+int launch_split_f64(hipStream_t s, const double *x, long n, uint16_t *hi, uint16_t *lo, float *f32);
+int launch_mx_operands(hipStream_t s, const double *b, const double *a, uint16_t *Bh, uint16_t *Bl, float *Bf,
+                       uint16_t *Ah, uint16_t *Al, float *Af);
+int launch_mx_gain(hipStream_t s, int mode, const uint16_t *Ah, const uint16_t *Al, const float *Af, const double *f,
+                   int cells, float *gpart);
+int launch_mx_combine(hipStream_t s, int cells, const float *nuf, int ldn, const float *gpart, double *nu,
+                      double *gain);
+
 // ---- uneven batches (batch_abi.hip, korobov_kernels.hip) -------------------
 // One velocity grid (class) of a batch: its stencil, the caller's f / Q of
 // all its cells, and the scratch of the current segment's cells (slots).

@@ -1,0 +1,361 @@
+// Batched analytic-matrix collision operator on the matrix cores, 16^3 box
+// (evolve_direct_colisions / evolve_inverse_colisions, collision_integral.cu:
+// 32-188, for many cells at once), FP32 accumulation, FP64 per-cell sums.
+//
+// Loss: nu[c][k] = sum_j b(|k - j|) f[c][j] -- one GEMM, F (cells x N) times
+// the Toeplitz matrix B (N x N) built from the b table.
+//
+// Gain: with s = v + v1 and g = |v - v1| per axis (same parity as s), the
+// reference sum is  gain[v] = sum_v1 G[g, s],
+//   G[g, s] = sum_n1 a[n1, g] T[n1, s],
+//   T[n1, s] = f(A)f(B) + f(A_x,B_y,B_z)f(B_x,A_y,A_z) + f(A_x,A_y,B_z)f(B_x,B_y,A_z)
+//              + f(A_x,B_y,A_z)f(B_x,A_y,B_z),
+// A = (s + n)/2, B = (s - n)/2 with n = 2 n1 + parity(s) (term dropped when
+// A > 15 or s - n < 0 on an axis, :88-131).  Per parity class of s, G is a
+// GEMM: (512 g x 512 n1) times (512 n1 x S_class s).  A block owns one cell
+// and one class: f in LDS (FP64), the a operand streamed in K slices, T built
+// on the fly for a tile of 128 s, 8 waves of 64 g x 128 s MFMA tiles; every
+// G element is then added to gain[(s + sigma g) / 2] for each sign pattern
+// sigma with both velocities on the grid (the pairs (v, v1) of the
+// reference), in an FP32 LDS copy of the cell's gain.  The block writes its
+// class's partial gain; the 8 partials are summed in FP64.  The LDS adds
+// come in no fixed order (FP32 rounding differences of ~1e-7 relative run to
+// run, far below the bf16 operand rounding this path is stated with).
+//
+// Operand precision (mode): bf16, bf16 split into hi + lo (C = Ah Bh + Ah Bl
+// + Al Bh), or fp32 MFMA.  The conservation correction and Q = gain - f nu
+// run in FP64.
+#include "gdt_internal.h"
+#include "kernel_common.h"
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+#define MX_N 16           // grid nodes per axis
+#define MX_NB 4096        // nodes
+#define MX_G 512          // relative velocities per parity class (rows)
+#define MX_K 512          // boxes n1 per class (contraction)
+#define MX_S 128          // s per tile (columns)
+#define MX_T 512          // threads: 8 waves of 64 g rows
+#define MX_KB 32          // bf16 K slice
+#define MX_KF 16          // fp32 K slice
+#define MX_PB 40          // bf16 LDS row pitch (elements)
+#define MX_PF 17          // fp32 LDS row pitch (elements)
+
+__device__ __forceinline__ uint16_t mx_bf16(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float mx_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// s index within class p (axis parity p_a): s_a = 2 i_a + p_a, i_a < 16 - p_a.
+__device__ __forceinline__ void mx_s_of(int p, int si, int &sx, int &sy, int &sz)
+{
+    const int px = (p >> 2) & 1, py = (p >> 1) & 1, pz = p & 1;
+    const int nz = MX_N - pz, ny = MX_N - py;
+    const int iz = si % nz, iy = (si / nz) % ny, ix = si / (nz * ny);
+    sx = 2 * ix + px;
+    sy = 2 * iy + py;
+    sz = 2 * iz + pz;
+}
+
+__device__ __forceinline__ int mx_s_count(int p)
+{
+    return (MX_N - ((p >> 2) & 1)) * (MX_N - ((p >> 1) & 1)) * (MX_N - (p & 1));
+}
+
+// T[n1, s] from the LDS copy of f (FP64), with the reference's drop rule.
+__device__ __forceinline__ double mx_T(const double *fs, int p, int sx, int sy, int sz, int n1)
+{
+    const int nx = 2 * (n1 >> 6) + ((p >> 2) & 1), ny = 2 * ((n1 >> 3) & 7) + ((p >> 1) & 1),
+              nz = 2 * (n1 & 7) + (p & 1);
+    const int ax = (sx + nx) >> 1, ay = (sy + ny) >> 1, az = (sz + nz) >> 1;
+    const int mx = sx - nx, my = sy - ny, mz = sz - nz;
+    if (ax > MX_N - 1 || mx < 0 || ay > MX_N - 1 || my < 0 || az > MX_N - 1 || mz < 0) return 0.0;
+    const int bx = mx >> 1, by = my >> 1, bz = mz >> 1;
+#define MF(x, y, z) fs[((x) * MX_N + (y)) * MX_N + (z)]
+    return MF(ax, ay, az) * MF(bx, by, bz) + MF(ax, by, bz) * MF(bx, ay, az) + MF(ax, ay, bz) * MF(bx, by, az) +
+           MF(ax, by, az) * MF(bx, ay, bz);
+#undef MF
+}
+
+// Adds G[g, s] (g = class row gr) to every output v = (s + sigma g) / 2 whose
+// partner v1 = (s - sigma g) / 2 is on the grid too.  Per axis both sign
+// choices are on the grid exactly when s >= g and (s + g) / 2 <= 15 (one
+// choice when g = 0), so validity is one test per element.
+__device__ __forceinline__ void mx_scatter(float *gl, int p, int gr, int sx, int sy, int sz, float val)
+{
+    const int gx = 2 * (gr >> 6) + ((p >> 2) & 1), gy = 2 * ((gr >> 3) & 7) + ((p >> 1) & 1),
+              gz = 2 * (gr & 7) + (p & 1);
+    if (sx < gx || sy < gy || sz < gz || sx + gx > 2 * (MX_N - 1) || sy + gy > 2 * (MX_N - 1) ||
+        sz + gz > 2 * (MX_N - 1))
+        return;
+    const int hx = (sx + gx) >> 1, lx = (sx - gx) >> 1, hy = (sy + gy) >> 1, ly = (sy - gy) >> 1,
+              hz = (sz + gz) >> 1, lz = (sz - gz) >> 1;
+    const int nx = gx ? 2 : 1, ny = gy ? 2 : 1, nz = gz ? 2 : 1;
+    for (int a = 0; a < nx; a++)
+        for (int b = 0; b < ny; b++)
+            for (int c = 0; c < nz; c++)
+                atomicAdd(&gl[((a ? lx : hx) * MX_N + (b ? ly : hy)) * MX_N + (c ? lz : hz)], val);
+}
+
+// MODE: 0 bf16, 1 bf16 split, 2 fp32.  Grid (8 classes, cells); AwT[p][g][n1]
+// (hi, lo for split; fp32 in Af for MODE 2); out gpart[c][p][v] (FP32).
+template <int MODE>
+__global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ Awh, const uint16_t *__restrict__ Awl,
+                                                  const float *__restrict__ Awf, const double *__restrict__ f,
+                                                  float *__restrict__ gpart)
+{
+    extern __shared__ __align__(16) unsigned char mx_lds[];
+    constexpr int NP = MODE == 1 ? 2 : 1;
+    double *fs = (double *)mx_lds;                       // 32 KB
+    float *gl = (float *)(mx_lds + 8 * MX_NB);           // 16 KB
+    unsigned char *ops = mx_lds + 12 * MX_NB;            // operand tiles
+    uint16_t *sA = (uint16_t *)ops;                      // [NP][512][MX_PB]
+    uint16_t *sB = sA + NP * MX_G * MX_PB;               // [NP][128][MX_PB]
+    float *fA = (float *)ops;                            // [512][MX_PF]
+    float *fB = fA + MX_G * MX_PF;                       // [128][MX_PF]
+    const int p = blockIdx.x, c = blockIdx.y;
+    const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+    const double *fc = f + (size_t)c * MX_NB;
+    for (int i = threadIdx.x; i < MX_NB; i += MX_T) {
+        fs[i] = fc[i];
+        gl[i] = 0.0f;
+    }
+    __syncthreads();
+    const int S = mx_s_count(p);
+    const size_t aoff = (size_t)p * MX_G * MX_K;
+    for (int s0 = 0; s0 < S; s0 += MX_S) {
+        f32x16 acc[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+        constexpr int KB = MODE == 2 ? MX_KF : MX_KB;
+        for (int k0 = 0; k0 < MX_K; k0 += KB) {
+            __syncthreads();  // previous slice's operand reads are done
+            if (MODE != 2) {
+                // a slice: 512 rows x 32 n1 = 2048 chunks of 8 bf16 per part
+#pragma unroll
+                for (int pp = 0; pp < NP; pp++) {
+                    const uint16_t *src = pp ? Awl : Awh;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int q = threadIdx.x + i * MX_T, row = q / 4, col = (q % 4) * 8;
+                        *(uint4 *)(&sA[(pp * MX_G + row) * MX_PB + col]) =
+                            *(const uint4 *)(src + aoff + (size_t)row * MX_K + k0 + col);
+                    }
+                }
+                // T slice: 128 s x 32 n1, 8 per thread
+#pragma unroll 2
+                for (int i = 0; i < 8; i++) {
+                    const int q = threadIdx.x + i * MX_T, sc = q / KB, nk = q % KB;
+                    double t = 0.0;
+                    if (s0 + sc < S) {
+                        int sx, sy, sz;
+                        mx_s_of(p, s0 + sc, sx, sy, sz);
+                        t = mx_T(fs, p, sx, sy, sz, k0 + nk);
+                    }
+                    const uint16_t h = mx_bf16((float)t);
+                    sB[sc * MX_PB + nk] = h;
+                    if (NP == 2) sB[(MX_S + sc) * MX_PB + nk] = mx_bf16((float)(t - (double)mx_f32(h)));
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int q = threadIdx.x + i * MX_T, row = q / 4, col = (q % 4) * 4;
+                    const float4 v = *(const float4 *)(Awf + aoff + (size_t)row * MX_K + k0 + col);
+                    float *d = &fA[row * MX_PF + col];
+                    d[0] = v.x;
+                    d[1] = v.y;
+                    d[2] = v.z;
+                    d[3] = v.w;
+                }
+#pragma unroll 2
+                for (int i = 0; i < 4; i++) {
+                    const int q = threadIdx.x + i * MX_T, sc = q / KB, nk = q % KB;
+                    double t = 0.0;
+                    if (s0 + sc < S) {
+                        int sx, sy, sz;
+                        mx_s_of(p, s0 + sc, sx, sy, sz);
+                        t = mx_T(fs, p, sx, sy, sz, k0 + nk);
+                    }
+                    fB[sc * MX_PF + nk] = (float)t;
+                }
+            }
+            __syncthreads();
+            if (MODE != 2) {
+#pragma unroll
+                for (int kk = 0; kk < MX_KB; kk += 16) {
+                    bf16x8 a[NP][2], b[NP][4];
+#pragma unroll
+                    for (int pp = 0; pp < NP; pp++) {
+#pragma unroll
+                        for (int i = 0; i < 2; i++)
+                            a[pp][i] = *(const bf16x8 *)(&sA[(pp * MX_G + w * 64 + i * 32 + l % 32) * MX_PB + kk +
+                                                             (l / 32) * 8]);
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            b[pp][j] = *(const bf16x8 *)(&sB[(pp * MX_S + j * 32 + l % 32) * MX_PB + kk + (l / 32) * 8]);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+                            if (NP == 2) {
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[NP - 1][j], acc[i][j],
+                                                                                   0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[NP - 1][i], b[0][j], acc[i][j],
+                                                                                   0, 0, 0);
+                            }
+                        }
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < MX_KF; kk += 2) {
+                    float a[2], b[4];
+#pragma unroll
+                    for (int i = 0; i < 2; i++) a[i] = fA[(w * 64 + i * 32 + l % 32) * MX_PF + kk + l / 32];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) b[j] = fB[(j * 32 + l % 32) * MX_PF + kk + l / 32];
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        // scatter the tile's G into the cell's gain: each 32 x 32 MFMA tile is
+        // staged in the wave's LDS slot (operand space, free after the K loop),
+        // then walked by a small runtime loop
+        float *stg = (float *)ops + w * 32 * 33;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                __syncthreads();  // operand reads / previous staging reads done
+#pragma unroll
+                for (int e = 0; e < 16; e++) stg[((e / 4) * 8 + (l / 32) * 4 + e % 4) * 33 + l % 32] = acc[i][j][e];
+                __syncthreads();
+                for (int e = 0; e < 16; e++) {
+                    const int q = l + e * WAVE, r = q / 32, cc = q % 32;
+                    const int sc = s0 + j * 32 + cc;
+                    const float val = stg[r * 33 + cc];
+                    if (sc < S && val != 0.0f) {
+                        int sx, sy, sz;
+                        mx_s_of(p, sc, sx, sy, sz);
+                        mx_scatter(gl, p, w * 64 + i * 32 + r, sx, sy, sz, val);
+                    }
+                }
+            }
+    }
+    __syncthreads();
+    float *out = gpart + ((size_t)c * 8 + p) * MX_NB;
+    for (int i = threadIdx.x; i < MX_NB; i += MX_T) out[i] = gl[i];
+}
+
+// Operands from the FP64 tables: B[k][j] = b(|k - j|) (hi, lo, fp32) and
+// AwT[p][g][n1] = a[in1, gi, jn1, kn1, gj, gk] with g of parity p.
+__global__ void k_mx_bmat(const double *__restrict__ b, uint16_t *__restrict__ Bh, uint16_t *__restrict__ Bl,
+                          float *__restrict__ Bf)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)MX_NB * MX_NB) return;
+    const int k = (int)(t / MX_NB), j = (int)(t % MX_NB);
+    const int gi = abs(k / 256 - j / 256), gj = abs((k / 16) % 16 - (j / 16) % 16), gk = abs(k % 16 - j % 16);
+    const double v = b[(gi * MX_N + gj) * MX_N + gk];
+    const uint16_t h = mx_bf16((float)v);
+    Bh[t] = h;
+    Bl[t] = mx_bf16((float)(v - (double)mx_f32(h)));
+    Bf[t] = (float)v;
+}
+
+__global__ void k_mx_awt(const double *__restrict__ a, uint16_t *__restrict__ Ah, uint16_t *__restrict__ Al,
+                         float *__restrict__ Af)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8 * MX_G * MX_K) return;
+    const int n1 = t % MX_K, gr = (t / MX_K) % MX_G, p = t / (MX_K * MX_G);
+    const int gi = 2 * (gr >> 6) + ((p >> 2) & 1), gj = 2 * ((gr >> 3) & 7) + ((p >> 1) & 1),
+              gk = 2 * (gr & 7) + (p & 1);
+    const int in1 = n1 >> 6, jn1 = (n1 >> 3) & 7, kn1 = n1 & 7;
+    const int hv = MX_N / 2, hw = MX_N / 2;
+    const long ai = ((long)((in1 * MX_N + gi) * (hv * hw) + jn1 * hw + kn1)) * (MX_N * MX_N) + gj * MX_N + gk;
+    const double v = a[ai];
+    const uint16_t h = mx_bf16((float)v);
+    Ah[t] = h;
+    Al[t] = mx_bf16((float)(v - (double)mx_f32(h)));
+    Af[t] = (float)v;
+}
+
+// FP64 sums: nu from the FP32 loss GEMM, gain from the 8 class partials.
+__global__ void k_mx_combine(long count, const float *__restrict__ nuf, int ldn, const float *__restrict__ gpart,
+                             double *__restrict__ nu, double *__restrict__ gain)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const long c = t / MX_NB, v = t % MX_NB;
+    nu[t] = (double)nuf[c * ldn + v];
+    double g = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; p++) g += (double)gpart[(c * 8 + p) * MX_NB + v];
+    gain[t] = g;
+}
+
+int launch_mx_operands(hipStream_t s, const double *b, const double *a, uint16_t *Bh, uint16_t *Bl, float *Bf,
+                       uint16_t *Ah, uint16_t *Al, float *Af)
+{
+    const long nb = (long)MX_NB * MX_NB;
+    hipLaunchKernelGGL(k_mx_bmat, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, b, Bh, Bl, Bf);
+    GDT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mx_awt, dim3((8 * MX_G * MX_K + 255) / 256), dim3(256), 0, s, a, Ah, Al, Af);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+static size_t mx_lds_bytes(int mode)
+{
+    const size_t base = 12 * MX_NB;
+    if (mode == GDT_GEMM_F32) return base + sizeof(float) * (MX_G + MX_S) * MX_PF;
+    const int np = mode == GDT_GEMM_BF16_SPLIT ? 2 : 1;
+    return base + sizeof(uint16_t) * np * (MX_G + MX_S) * MX_PB;
+}
+
+int launch_mx_gain(hipStream_t s, int mode, const uint16_t *Ah, const uint16_t *Al, const float *Af, const double *f,
+                   int cells, float *gpart)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        GDT_HIP(hipFuncSetAttribute((const void *)k_mx_gain<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)mx_lds_bytes(GDT_GEMM_BF16)));
+        GDT_HIP(hipFuncSetAttribute((const void *)k_mx_gain<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)mx_lds_bytes(GDT_GEMM_BF16_SPLIT)));
+        GDT_HIP(hipFuncSetAttribute((const void *)k_mx_gain<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)mx_lds_bytes(GDT_GEMM_F32)));
+        attr_set = true;
+    }
+    const dim3 grid(8, cells);
+    if (mode == GDT_GEMM_BF16)
+        hipLaunchKernelGGL(k_mx_gain<0>, grid, dim3(MX_T), mx_lds_bytes(mode), s, Ah, Ah, Af, f, gpart);
+    else if (mode == GDT_GEMM_BF16_SPLIT)
+        hipLaunchKernelGGL(k_mx_gain<1>, grid, dim3(MX_T), mx_lds_bytes(mode), s, Ah, Al, Af, f, gpart);
+    else
+        hipLaunchKernelGGL(k_mx_gain<2>, grid, dim3(MX_T), mx_lds_bytes(mode), s, Ah, Al, Af, f, gpart);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+int launch_mx_combine(hipStream_t s, int cells, const float *nuf, int ldn, const float *gpart, double *nu,
+                      double *gain)
+{
+    const long count = (long)cells * MX_NB;
+    hipLaunchKernelGGL(k_mx_combine, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, count, nuf, ldn, gpart,
+                       nu, gain);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}

@@ -13,7 +13,7 @@
  * calling thread's last failure.
  *
  * Each entry point names the reference interface it replaces.  The C++
- * drop-in layer (include/gpu_determ/*.h, libgpu_determ.so) keeps the
+ * drop-in layer (the headers under include/gpu_determ, libgpu_determ.so) keeps the
  * reference's own names and signatures and is built on these calls.
  */
 #ifndef GDT_H
@@ -156,6 +156,28 @@ int gdt_relax(gdt_handle h, double *f, const double *nu, const double *gain, int
  * each: collide, correct (if flags & GDT_FLAG_CORRECT), f += dt Q.  With
  * GDT_FLAG_GRAPH the step is captured once as a hipGraph and replayed. */
 int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int steps, unsigned flags);
+
+/* ---- matrix cores (no reference counterpart) ----------------------------- */
+#define GDT_GEMM_BF16 0       /* bf16 operands                                   */
+#define GDT_GEMM_BF16_SPLIT 1 /* operands split into bf16 hi + lo, 3 products    */
+#define GDT_GEMM_F32 2        /* fp32 operands (fp32 MFMA)                       */
+/* C[M][N] (+)= sum_k A[M][K] B[N][K] on MFMA with FP32 accumulation; device
+ * operands (bf16 as uint16 bit patterns, or float), M and N multiples of 128,
+ * K a multiple of 32 (bf16) / 16 (fp32).  Default stream, synchronous. */
+int gdt_gemm_nt(int mode, int M, int N, int K, const void *A_hi, const void *A_lo, const void *B_hi,
+                const void *B_lo, float *C, int ldc, int accumulate);
+/* FP64 -> bf16 pair (hi = bf16(x), lo = bf16(x - hi)) and / or FP32; any
+ * output may be NULL.  Default stream, synchronous. */
+int gdt_split_f64(const double *x, long n, uint16_t *hi, uint16_t *lo, float *f32);
+/* The analytic-matrix operator of evolve_colisions (collision_integral.cu:
+ * 550-773) for many cells of a 16^3 box grid (R keeping every node, tables
+ * built), as dense contractions on the matrix cores: the loss a GEMM against
+ * the Toeplitz b matrix, the gain a GEMM per parity class of v + v1 over the
+ * a table (mode: GDT_GEMM_*), FP32 accumulation; FP64 per-cell sums, Q and
+ * conservation correction (flags & GDT_FLAG_CORRECT).  f, nu, gain, Q:
+ * device FP64 [cells][4096]; outputs as gdt_evaluate's (nu corrected). */
+int gdt_matrix_mfma(gdt_handle h, int mode, const double *f, double *nu, double *gain, double *Q, int cells,
+                    unsigned flags);
 
 /* ---- uneven batches (no reference counterpart: cells of several grids) ---
  * Classes are handles with a Korobov stencil built; f[s] is device FP64
