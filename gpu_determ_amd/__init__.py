@@ -42,6 +42,9 @@ ABI_SYMBOLS = (
     "gdt_batch_create", "gdt_batch_destroy", "gdt_batch_set_output", "gdt_batch_config", "gdt_batch_tune",
     "gdt_batch_run", "gdt_batch_synchronize", "gdt_batch_info",
     "gdt_gemm_nt", "gdt_split_f64", "gdt_matrix_mfma",
+    "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
+    "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
+    "gdt_init_mixture",
 )
 
 _lib = None
@@ -113,6 +116,15 @@ def lib():
         "gdt_gemm_nt": [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int],
         "gdt_split_f64": [vp, C.c_long, vp, vp, vp],
         "gdt_matrix_mfma": [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_uint],
+        "gdt_surrogate_create": [C.POINTER(vp), vp, C.c_int, C.c_uint64],
+        "gdt_surrogate_destroy": [vp],
+        "gdt_surrogate_train": [vp, vp, vp, C.c_long, C.c_int, C.c_int, C.c_double, dp],
+        "gdt_surrogate_predict": [vp, vp, vp, C.c_int],
+        "gdt_surrogate_infer": [vp, vp, vp, C.c_int, C.c_double, vp, ip],
+        "gdt_surrogate_residuals": [vp, vp, C.c_int],
+        "gdt_surrogate_export": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "gdt_surrogate_info": [vp, ip, ip, ip, C.POINTER(C.c_long)],
+        "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -293,6 +305,16 @@ class Context:
     def run_steps(self, method, f, cells, dt, steps, flags=FLAG_CORRECT):
         check(lib().gdt_run_steps(self.h, int(method), f.ptr, int(cells), float(dt), int(steps), int(flags)))
 
+    def init_mixture(self, params):
+        """f [cells][n_pnt] (DeviceArray) from params [cells][comps][5] = (n, T, ux, uy, uz)."""
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        cells, comps = params.shape[0], params.shape[1]
+        pd = DeviceArray.from_host(params)
+        f = DeviceArray((cells, self.n_pnt))
+        check(lib().gdt_init_mixture(self.h, pd.ptr, comps, cells, f.ptr))
+        self.sync()
+        return f
+
     def matrix_mfma(self, mode, f, cells, flags=FLAG_CORRECT):
         """Batched analytic-matrix operator on the matrix cores (16^3 box):
         returns (nu corrected, gain, Q) DeviceArrays [cells][4096]."""
@@ -355,6 +377,61 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().gdt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Surrogate:
+    """Learned Q(f,f) surrogate on the grid of a Context (gdt_surrogate_*)."""
+
+    def __init__(self, ctx, hidden=128, seed=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().gdt_surrogate_create(C.byref(h), ctx.h, int(hidden), int(seed)))
+        self.h = h
+        n, np_, H, P = C.c_int(), C.c_int(), C.c_int(), C.c_long()
+        check(lib().gdt_surrogate_info(self.h, C.byref(n), C.byref(np_), C.byref(H), C.byref(P)))
+        self.n, self.np, self.hidden, self.params = n.value, np_.value, H.value, P.value
+
+    def train(self, f, Q, epochs=10, batch=256, lr=1e-3):
+        loss = C.c_double()
+        check(lib().gdt_surrogate_train(self.h, f.ptr, Q.ptr, int(f.shape[0]), int(epochs), int(batch), float(lr),
+                                        C.byref(loss)))
+        return loss.value
+
+    def predict(self, f):
+        Q = DeviceArray(f.shape)
+        check(lib().gdt_surrogate_predict(self.h, f.ptr, Q.ptr, int(f.shape[0])))
+        return Q
+
+    def infer(self, f, threshold=1e-4):
+        cells = int(f.shape[0])
+        Q = DeviceArray(f.shape)
+        fb = DeviceArray((cells,), np.int32)
+        nfb = C.c_int()
+        check(lib().gdt_surrogate_infer(self.h, f.ptr, Q.ptr, cells, float(threshold), fb.ptr, C.byref(nfb)))
+        res = np.empty(cells)
+        check(lib().gdt_surrogate_residuals(self.h, res.ctypes.data, cells))
+        return Q, fb.get().astype(bool), res
+
+    def export(self):
+        H, n_ = self.hidden, self.np
+        W1, b1 = np.empty((H, n_), np.float32), np.empty(H, np.float32)
+        W2, b2 = np.empty((H, n_), np.float32), np.empty(H, np.float32)
+        W3, b3 = np.empty((n_, H), np.float32), np.empty(n_, np.float32)
+        norm = np.empty((4, n_), np.float32)
+        check(lib().gdt_surrogate_export(self.h, W1.ctypes.data, b1.ctypes.data, W2.ctypes.data, b2.ctypes.data,
+                                         W3.ctypes.data, b3.ctypes.data, norm.ctypes.data))
+        return W1, b1, W2, b2, W3, b3, norm
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gdt_surrogate_destroy(self.h)
             self.h = None
 
     def __del__(self):

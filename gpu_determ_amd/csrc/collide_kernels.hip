@@ -501,3 +501,32 @@ int launch_fill(gdt_context *h, double *dst, long count, double value)
     return GDT_OK;
 }
 
+
+// f[c][v] = sum_k n_k (2 pi T_k)^(-3/2) exp(-|xi_v - u_k|^2 / (2 T_k)),
+// params[c][k] = (n, T, ux, uy, uz); components with n = 0 contribute nothing.
+__global__ void k_init_mixture(int npts, int comps, long cells, const double *__restrict__ xi,
+                               const double *__restrict__ params, double *__restrict__ f)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells * npts) return;
+    const long c = t / npts;
+    const int v = (int)(t % npts);
+    const double x = xi[3 * v], y = xi[3 * v + 1], z = xi[3 * v + 2];
+    double s = 0.0;
+    for (int k = 0; k < comps; k++) {
+        const double *p = params + (c * comps + k) * 5;
+        if (p[0] == 0.0) continue;
+        const double dx = x - p[2], dy = y - p[3], dz = z - p[4];
+        s += p[0] * pow(2.0 * M_PI * p[1], -1.5) * exp(-(dx * dx + dy * dy + dz * dz) / (2.0 * p[1]));
+    }
+    f[t] = s;
+}
+
+int launch_init_mixture(gdt_context *h, const double *params, int comps, long cells, double *f)
+{
+    const long count = cells * h->n_pnt;
+    hipLaunchKernelGGL(k_init_mixture, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->stream, h->n_pnt,
+                       comps, cells, h->xi.as<double>(), params, f);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}

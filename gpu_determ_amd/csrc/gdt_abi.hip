@@ -932,3 +932,10 @@ int gdt_transpose(gdt_handle h, const double *src, double *dst, int rows, int co
 }
 
 }  // extern "C"
+
+int gdt_init_mixture(gdt_handle h, const double *params, int comps, long cells, double *f)
+{
+    H_CHECK(h);
+    GDT_CHECK(params && f && comps > 0 && cells > 0, GDT_E_ARG, "gdt_init_mixture: bad arguments");
+    return launch_init_mixture(h, params, comps, cells, f);
+}

@@ -183,3 +183,4 @@ int launch_euler(gdt_context *h, double *f, const double *nu, const double *gain
 int launch_convert(const float *s32, double *d64, size_t count, const double *s64, float *d32);
 int launch_basis(gdt_context *h, double *dst);
 int launch_fill(gdt_context *h, double *dst, long count, double value);
+int launch_init_mixture(gdt_context *h, const double *params, int comps, long cells, double *f);

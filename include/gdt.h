@@ -179,6 +179,38 @@ int gdt_split_f64(const double *x, long n, uint16_t *hi, uint16_t *lo, float *f3
 int gdt_matrix_mfma(gdt_handle h, int mode, const double *f, double *nu, double *gain, double *Q, int cells,
                     unsigned flags);
 
+/* ---- learned surrogate (no reference counterpart) ------------------------
+ * Bilinear MLP on the node values of the grid of h: x = standardised f / n,
+ * h = (W1 x + b1) * (W2 x + b2) elementwise, W3 h + b3 -> standardised
+ * Q / n^2 (n = density), bf16 weights and activations on MFMA with FP32
+ * accumulation, FP32 master weights and Adam; hidden a multiple of 128.
+ * The handle h must outlive the surrogate. */
+typedef struct gdt_surrogate *gdt_surrogate_handle;
+int gdt_surrogate_create(gdt_surrogate_handle *s, gdt_handle h, int hidden, uint64_t seed);
+int gdt_surrogate_destroy(gdt_surrogate_handle s);
+/* Trains on ns pairs (device FP64 f, Q [ns][n_pnt]); loss_out: mean square
+ * error (standardised units) of the last sampled step.  Synchronous. */
+int gdt_surrogate_train(gdt_surrogate_handle s, const double *f, const double *Q, long ns, int epochs, int batch,
+                        double lr, double *loss_out);
+/* Network prediction only (no projection): Q [cells][n_pnt] device FP64. */
+int gdt_surrogate_predict(gdt_surrogate_handle s, const double *f, double *Q, int cells);
+/* Prediction, exact conservation projection (FP64 5x5 per cell, weights f),
+ * residual r = |Q - Q_nn| / |Q| (1 if sum Q log f > 0) and, for r >
+ * threshold, the plain solver (corrected Korobov evaluation on the stencil of
+ * h) in place of the network.  fallback (device int [cells], may be NULL)
+ * marks those cells; *n_fallback counts them.  Synchronous. */
+int gdt_surrogate_infer(gdt_surrogate_handle s, const double *f, double *Q, int cells, double threshold,
+                        int *fallback, int *n_fallback);
+/* Host copy of the residuals of the last gdt_surrogate_infer call. */
+int gdt_surrogate_residuals(gdt_surrogate_handle s, double *res, int cells);
+/* Host copies of the FP32 master weights ([hidden][np], [hidden],
+ * [hidden][np], [hidden], [np][hidden], [np]) and the standardisation
+ * (mu_x, sd_x, mu_y, sd_y, each [np]); any may be NULL.  np = n_pnt rounded
+ * up to 128. */
+int gdt_surrogate_export(gdt_surrogate_handle s, float *W1, float *b1, float *W2, float *b2, float *W3, float *b3,
+                         float *norm);
+int gdt_surrogate_info(gdt_surrogate_handle s, int *n, int *np, int *hidden, long *params);
+
 /* ---- uneven batches (no reference counterpart: cells of several grids) ---
  * Classes are handles with a Korobov stencil built; f[s] is device FP64
  * [cells[s]][n_pnt of class s] (cell-major), updated in place by steps.
@@ -202,6 +234,13 @@ int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node
 int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags);
 int gdt_batch_synchronize(gdt_batch_handle b);
 int gdt_batch_info(gdt_batch_handle b, int *seg_cells, int *node_q, int *segments);
+
+/* ---- initial conditions ---------------------------------------------------- */
+/* Sums of Maxwellians on the handle's grid (the reference's init_f is one
+ * component, determ.cu:101-121): f[c][v] = sum_k n (2 pi T)^(-3/2)
+ * exp(-|xi_v - u|^2 / 2T) with params[c][k] = (n, T, ux, uy, uz), device FP64;
+ * a component with n = 0 is skipped.  Asynchronous on the handle's stream. */
+int gdt_init_mixture(gdt_handle h, const double *params, int comps, long cells, double *f);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Kernel timing under GDT_FLAG_TIME: total milliseconds and launch count of

@@ -293,3 +293,19 @@ def test_korobov_relaxation_cfg2_short():
     assert np.max(np.abs(m1 - m0) / np.abs(m0).max()) < 1e-13
     H0 = float(np.sum(f0 * np.log(f0)) * g.d3v)
     assert all(b <= a + 1e-12 for a, b in zip([H0] + H[:-1], H))
+
+
+def test_init_mixture_matches_numpy():
+    """gdt_init_mixture against the closed form on the host (n = 0 components skipped)."""
+    say("init mixture")
+    from helpers import maxwellian
+    ctx = G.Context((12, 12, 12), (-4.8,) * 3, (4.8,) * 3, 4.8)
+    rng = np.random.default_rng(9)
+    cells, comps = 7, 3
+    par = np.zeros((cells, comps, 5))
+    par[:, :, 0] = rng.uniform(0.2, 1.0, (cells, comps)) * (rng.uniform(0, 1, (cells, comps)) > 0.3)
+    par[:, :, 1] = rng.uniform(0.3, 1.5, (cells, comps))
+    par[:, :, 2:] = rng.uniform(-1, 1, (cells, comps, 3))
+    f = ctx.init_mixture(par).get()
+    ref = np.stack([sum(maxwellian(ctx.xi, *par[c, k, :2], par[c, k, 2:]) for k in range(comps)) for c in range(cells)])
+    assert rel_l2(f, ref) < 1e-14
