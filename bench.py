@@ -320,8 +320,9 @@ def main():
     n = ctx.n_pnt
     nn, ne = info["nodes"], info["entries"]
     # algorithmic bytes per launch (cells = 1)
-    G_ = 8        # node groups of the gather (one per XCD)
-    nparts = 512  # gather blocks per cell: 8 groups x 64 tiles at this size
+    tl = ctx.korobov_tiles()
+    G_ = tl["groups"]              # node groups of the gather (one per XCD)
+    nparts = G_ * tl["tiles"]      # gather tiles per cell: one moment partial each
     alg = {
         # node records (uint4 + D2) + f in; (D, 2wP) per node out
         "kor_node": nn * 20 + 8 * n + nn * 16,

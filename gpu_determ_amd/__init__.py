@@ -44,7 +44,7 @@ ABI_SYMBOLS = (
     "gdt_gemm_nt", "gdt_split_f64", "gdt_matrix_mfma",
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
-    "gdt_init_mixture",
+    "gdt_init_mixture", "gdt_korobov_tiles",
 )
 
 _lib = None
@@ -125,6 +125,7 @@ def lib():
         "gdt_surrogate_export": [vp, vp, vp, vp, vp, vp, vp, vp],
         "gdt_surrogate_info": [vp, ip, ip, ip, C.POINTER(C.c_long)],
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
+        "gdt_korobov_tiles": [vp, ip, ip, ip],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -254,6 +255,11 @@ class Context:
         K = C.c_double()
         check(lib().gdt_korobov_info(self.h, C.byref(M), C.byref(a), C.byref(nodes), C.byref(ents), C.byref(K)))
         return {"M": M.value, "a": a.value, "nodes": nodes.value, "entries": ents.value, "K": K.value}
+
+    def korobov_tiles(self):
+        g, t, cap = C.c_int(), C.c_int(), C.c_int()
+        check(lib().gdt_korobov_tiles(self.h, C.byref(g), C.byref(t), C.byref(cap)))
+        return {"groups": g.value, "tiles": t.value, "cap": cap.value}
 
     def korobov_export(self):
         info = self.korobov_info()

@@ -510,11 +510,11 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
         }
         // gather tiles: per group, contiguous destination ranges of at most
         // KOR_TILE_DST destinations and about 1/NT of the group's entries; NT
-        // is the largest count (<= 64) for which every group needs at most 64
-        // tiles, so that G * 64 blocks, two per CU, fill the chip in one round
+        // is the largest count (<= KOR_TILES_MAX) for which every group needs at
+        // most KOR_TILES_MAX tiles (G * KOR_TILES_MAX blocks, two per CU)
         std::vector<int4> tiles;
         int ntile = 0, maxe = 0;
-        for (int NT = 64; NT >= 1; NT--) {
+        for (int NT = KOR_TILES_MAX; NT >= 1; NT--) {
             std::vector<std::vector<int4>> per(G);
             ntile = 0;
             maxe = 0;
@@ -531,7 +531,7 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 }
                 ntile = std::max(ntile, (int)per[g].size());
             }
-            if (ntile <= 64 || NT == 1) {
+            if (ntile <= KOR_TILES_MAX || NT == 1) {
                 tiles.assign((size_t)G * ntile, make_int4(0, 0, 0, 0));
                 for (int g = 0; g < G; g++)
                     for (size_t k = 0; k < per[g].size(); k++) tiles[(size_t)g * ntile + k] = per[g][k];
@@ -638,7 +638,7 @@ static bool use_fused(unsigned flags) { return (flags & GDT_FLAG_FUSED) != 0; }
 // group partials, and the fused path's wave partition (outside any capture).
 static int kor_prepare(gdt_context *h, int cells)
 {
-    const int nb = std::max(kor_blocks(h, cells), kor_gather_blocks(h, cells));
+    const int nb = std::max(kor_blocks(h, cells), kor_gather_parts(h));
     const size_t mp = sizeof(double) * 20 * (size_t)cells * nb;
     const size_t dw = sizeof(double) * 2 * (size_t)h->nn * cells;
     const size_t pt = sizeof(double) * 2 * (size_t)KOR_NGROUPS * h->n_pnt * cells;
@@ -666,7 +666,7 @@ static int kor_partials(gdt_context *h, const double *f_cm, int cells, unsigned 
         return launch_kor_fused(h, f_cm, kp.L, kp.Q, h->s_mpart.as<double>(), cells);
     }
     kp.G = KOR_NGROUPS;
-    kp.nparts = kor_gather_blocks(h, cells);
+    kp.nparts = kor_gather_parts(h);
     kp.L = h->s_part.as<double>();
     kp.Q = kp.L + (size_t)KOR_NGROUPS * cells * h->n_pnt;
     if ((rc = launch_kor_node(h, f_cm, h->s_vals.as<double>(), cells))) return rc;
@@ -938,4 +938,14 @@ int gdt_init_mixture(gdt_handle h, const double *params, int comps, long cells, 
     H_CHECK(h);
     GDT_CHECK(params && f && comps > 0 && cells > 0, GDT_E_ARG, "gdt_init_mixture: bad arguments");
     return launch_init_mixture(h, params, comps, cells, f);
+}
+
+int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_korobov_tiles: stencil not built");
+    if (groups) *groups = KOR_NGROUPS;
+    if (tiles) *tiles = h->kntile;
+    if (cap) *cap = h->kcap;
+    return GDT_OK;
 }

@@ -253,14 +253,30 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
             }
         }
         __syncthreads();
+        // four interleaved partial sums per destination (loads in flight
+        // together), combined in a fixed order: reproducible
 #pragma unroll
         for (int i = 0; i < TILE_DPT; i++) {
             const int a = max(ea[i], w0), bb = min(eb[i], w1);
-            for (int e = a; e < bb; e++) {
-                const double2 t = ql[e - w0];
-                q[i] += t.x;
-                l[i] += t.y;
+            double q4[4] = {0.0, 0.0, 0.0, 0.0}, l4[4] = {0.0, 0.0, 0.0, 0.0};
+            int e = a;
+            for (; e + 4 <= bb; e += 4) {
+                double2 t[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) t[r] = ql[e + r - w0];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    q4[r] += t[r].x;
+                    l4[r] += t[r].y;
+                }
             }
+            for (int r = 0; e < bb; e++, r++) {
+                const double2 t = ql[e - w0];
+                q4[r] += t.x;
+                l4[r] += t.y;
+            }
+            q[i] += (q4[0] + q4[1]) + (q4[2] + q4[3]);
+            l[i] += (l4[0] + l4[1]) + (l4[2] + l4[3]);
         }
     }
     double m[20];
@@ -652,10 +668,13 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
     return GDT_OK;
 }
 
+int kor_gather_parts(gdt_context *h) { return KOR_GROUPS * h->kntile; }
+
+// one block per tile (two per CU: the G * 64 tiles of a cell run in one round)
 int kor_gather_blocks(gdt_context *h, int cells)
 {
     (void)cells;
-    return KOR_GROUPS * h->kntile;
+    return kor_gather_parts(h);
 }
 
 int launch_kor_gather(gdt_context *h, const double *V, double *Lpart, double *Qpart, double *mpart, int cells)

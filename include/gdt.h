@@ -120,6 +120,9 @@ int gdt_korobov_info(gdt_handle h, uint32_t *M, uint32_t *a, long *nodes, long *
 /* Host copies of the stencil: idx6[nodes][6] (alpha, beta, lambda, mu,
  * lambda+s, mu-s), r[nodes], w[nodes], node[nodes] (lattice number). */
 int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32_t *node);
+/* Gather layout of the stencil: node groups (one per XCD), destination
+ * tiles per group (one moment partial each), LDS entries per gather pass. */
+int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
 /* Host copy of the CSR: ptr[n_pnt+1], ent[entries] = node*4 + role. */
 int gdt_korobov_export_csr(gdt_handle h, int32_t *ptr, uint32_t *ent);
 /* Generator choice alone (host): same rule as gdt_korobov_build. */
