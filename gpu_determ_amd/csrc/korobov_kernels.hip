@@ -116,12 +116,21 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
     double Da, Wa, ra, Db, Wb, rb;
     node_terms(xa, ya, fs, K, Da, Wa, ra);
     node_terms(xb, yb, fs, K, Db, Wb, rb);
-    if (ta < nn) out[ta] = make_double2(Da, Wa);
-    if (tb < nn) out[tb] = make_double2(Db, Wb);
+    // streaming (nontemporal) stores: the values are read by the next kernel
+    // only, from other XCDs, so they need not linger in this XCD's L2
+    if (ta < nn) {
+        __builtin_nontemporal_store(Da, &out[ta].x);
+        __builtin_nontemporal_store(Wa, &out[ta].y);
+    }
+    if (tb < nn) {
+        __builtin_nontemporal_store(Db, &out[tb].x);
+        __builtin_nontemporal_store(Wb, &out[tb].y);
+    }
     for (long t = tb + stride; t < nn; t += stride) {
         double D, W2P, r;
         node_terms(nrec[t], nd2[t], fs, K, D, W2P, r);
-        out[t] = make_double2(D, W2P);
+        __builtin_nontemporal_store(D, &out[t].x);
+        __builtin_nontemporal_store(W2P, &out[t].y);
     }
 }
 
@@ -287,8 +296,8 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
         const int d = d0 + threadIdx.x + i * TILE_THREADS;
         if (d < d1) {
             const size_t po = (size_t)g * npts + d;
-            Qp[po] = q[i];
-            Lp[po] = l[i];
+            __builtin_nontemporal_store(q[i], &Qp[po]);
+            __builtin_nontemporal_store(l[i], &Lp[po]);
             moments_add_psi(m, px[i], py[i], pz[i], q[i], l[i]);
         }
     }
