@@ -461,10 +461,19 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
             o4[2] = (uint32_t)o[4] | ((uint32_t)o[5] << 16);
             o4[3] = ((uint32_t)cint[3 * t] & 0xFFFFu) | (((uint32_t)cint[3 * t + 1] & 0xFFFFu) << 16);
         };
+        // Node order of the packed records (node values and gather): sorted
+        // by the lambda destination, lattice order among equals.  The entries
+        // a destination receives as lambda then come from one contiguous run
+        // of node values (coalesced loads in the gather), and its lambda+s
+        // entries from the runs of its neighbours.
+        std::vector<uint32_t> order(nn);
+        for (long t = 0; t < nn; t++) order[t] = (uint32_t)t;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t x, uint32_t y) { return cidx6[6 * (size_t)x + 2] < cidx6[6 * (size_t)y + 2]; });
         std::vector<uint32_t> n4(4 * (size_t)nn), n1(nn);
-        for (long t = 0; t < nn; t++) {
-            pack(t, &n4[4 * t]);
-            n1[t] = (uint32_t)cint[3 * t + 2];
+        for (long p = 0; p < nn; p++) {
+            pack(order[p], &n4[4 * p]);
+            n1[p] = (uint32_t)cint[3 * (size_t)order[p] + 2];
         }
         std::vector<int32_t> dest_of(ne);
         for (int v = 0; v < np; v++)
@@ -483,13 +492,16 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
         GDT_HIP(hipMemcpy(h->kn1.p, n1.data(), sizeof(uint32_t) * nn, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke4.p, e4.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke1.p, e1.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
-        // per node-group CSRs of the gather: group of node t is
-        // t * KOR_NGROUPS / nn (contiguous node ranges); within a group the
-        // entries are sorted by destination, then by (node, slot)
+        // per node-group CSRs of the gather over the sorted node positions p:
+        // group of p is p * KOR_NGROUPS / nn (contiguous ranges); within a
+        // group the entries are sorted by destination, then by (p, slot)
         const int G = KOR_NGROUPS;
         std::vector<int32_t> gp((size_t)G * (np + 1), 0);
-        auto grp = [&](uint32_t t) { return (int)(((uint64_t)t * G) / (uint64_t)nn); };
-        for (long e = 0; e < ne; e++) gp[(size_t)grp(ent[e] >> 2) * (np + 1) + dest_of[e] + 1]++;
+        auto grp = [&](uint32_t p) { return (int)(((uint64_t)p * G) / (uint64_t)nn); };
+        for (long p = 0; p < nn; p++) {
+            const long t = order[p];
+            for (int sl = 0; sl < slots(t); sl++) gp[(size_t)grp(p) * (np + 1) + cidx6[6 * t + sl] + 1]++;
+        }
         long run = 0;
         for (int g = 0; g < G; g++) {
             int32_t *pg = &gp[(size_t)g * (np + 1)];
@@ -499,14 +511,18 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 pg[v + 1] = (int32_t)run;
             }
         }
-        // entry records (node << 2 | role, num | den << 16 of the node)
+        // entry records (p << 2 | role, num | den << 16 of the node)
         std::vector<uint32_t> gent(2 * (size_t)ne);
         std::vector<int32_t> fill(gp);
-        for (long e = 0; e < ne; e++) {  // ent is destination-sorted with (node, slot) order inside
-            const int g = grp(ent[e] >> 2);
-            const int32_t pos = fill[(size_t)g * (np + 1) + dest_of[e]]++;
-            gent[2 * (size_t)pos] = ent[e];
-            gent[2 * (size_t)pos + 1] = n4[4 * (size_t)(ent[e] >> 2) + 3];
+        for (long p = 0; p < nn; p++) {
+            const long t = order[p];
+            const int g = grp(p);
+            for (int sl = 0; sl < slots(t); sl++) {
+                const uint32_t role = sl < 2 ? 0u : (sl < 4 ? 1u : 2u);
+                const int32_t pos = fill[(size_t)g * (np + 1) + cidx6[6 * t + sl]]++;
+                gent[2 * (size_t)pos] = ((uint32_t)p << 2) | role;
+                gent[2 * (size_t)pos + 1] = n4[4 * (size_t)p + 3];
+            }
         }
         // gather tiles: per group, contiguous destination ranges of at most
         // KOR_TILE_DST destinations and about 1/NT of the group's entries; NT
