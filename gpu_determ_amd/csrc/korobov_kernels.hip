@@ -429,6 +429,9 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // and solves the same 5x5 system; then each thread sums its node's G group
 // partials in group order and writes
 //   Q = Q0 - L X.psi,  loss = L,  nu = (L/f)(1 + X.psi),  f <- f + dt Q.
+// All global loads of a thread (its node's partials, f and centre, and its
+// share of the moment sums) are issued together at the start, so the block
+// waits for memory once.
 #define APPLY_ROUNDS 2  // block sums per thread and moment with loads in flight (512 blocks: one round)
 #define APPLY_THREADS 256
 // Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
@@ -442,28 +445,19 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
                                             double *__restrict__ Xout, double dt, double *f_upd, int b)
 {
     const int v = b * APPLY_THREADS + threadIdx.x;
-    double x = 0.0, y = 0.0, z = 0.0, L = 0.0, fv = 0.0, q0 = 0.0;
-    if (v < npts) {
-        x = xi[3 * v];
-        y = xi[3 * v + 1];
-        z = xi[3 * v + 2];
-        fv = fc[v];
-        double qg[G], lg[G];
+    const int vc = v < npts ? v : npts - 1;  // clamped: the loads are unconditional
+    const double x = xi[3 * vc], y = xi[3 * vc + 1], z = xi[3 * vc + 2], fv = fc[vc];
+    double qg[G], lg[G];
 #pragma unroll
-        for (int g = 0; g < G; g++) {
-            qg[g] = Qp[(size_t)g * npts + v];
-            lg[g] = Lp[(size_t)g * npts + v];
-        }
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            q0 += qg[g];
-            L += lg[g];
-        }
+    for (int g = 0; g < G; g++) {
+        qg[g] = Qp[(size_t)g * npts + vc];
+        lg[g] = Lp[(size_t)g * npts + vc];
     }
     if (CORR) {
         // block sums of the moments: thread t adds blocks t, t + 256, ...
-        // (coalesced, all loads in flight), then DPP row sums and the 16
-        // row sums in a fixed order (every block gets the same X)
+        // (coalesced, all loads in flight), then DPP row sums; wave 0 adds
+        // the 16 row sums in a fixed order (lane a: moment a) and solves
+        // (every block gets the same X)
         double m[20];
 #pragma unroll
         for (int a = 0; a < 20; a++) m[a] = 0.0;
@@ -492,15 +486,18 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
 #pragma unroll
             for (int a = 0; a < 20; a++) red[threadIdx.x / 16][a] = m[a];
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < WAVE) {
+            const int a = threadIdx.x < 20 ? threadIdx.x : 0;
+            double t = 0.0;
+#pragma unroll
+            for (int r = 0; r < APPLY_THREADS / 16; r++) t += red[r][a];
             double sm[20], X[5];
-            for (int a = 0; a < 20; a++) {
-                double t = 0.0;
-                for (int r = 0; r < APPLY_THREADS / 16; r++) t += red[r][a];
-                sm[a] = t;
-            }
+#pragma unroll
+            for (int k = 0; k < 20; k++) sm[k] = readlane_f64(t, k);
             moments_solve(sm, X);
-            for (int a = 0; a < 5; a++) Xs[a] = X[a];
+            if (threadIdx.x == 0)
+#pragma unroll
+                for (int k = 0; k < 5; k++) Xs[k] = X[k];
         }
     } else if (threadIdx.x < 5) {
         Xs[threadIdx.x] = 0.0;
@@ -508,6 +505,12 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
     __syncthreads();
     if (Xout && b == 0 && threadIdx.x < 5) Xout[threadIdx.x] = Xs[threadIdx.x];
     if (v < npts) {  // no early return: persistent callers run further items after this one
+        double q0 = 0.0, L = 0.0;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            q0 += qg[g];
+            L += lg[g];
+        }
         const double e = x * x + y * y + z * z;
         const double s = Xs[0] + Xs[1] * x + Xs[2] * y + Xs[3] * z + Xs[4] * e;
         const double q = q0 - L * s;
