@@ -128,11 +128,13 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
         sg.n_gather = (int)gi.size();
         sg.n_apply = (int)ai.size();
         // persistent grids: node blocks hold f in LDS (one or two per CU), gather
-        // blocks two per CU in a multiple of the group count, apply four per CU
+        // blocks as many per CU as their LDS allows (at most four) in a multiple
+        // of the group count, apply four per CU
         const int occ_node = std::max(1, std::min(2, (160 * 1024) / std::max(1, b->lds_node)));
+        const int occ_gather = std::max(1, std::min(4, (160 * 1024) / std::max(1, b->lds_gather)));
         sg.b_node = std::max(1, std::min(sg.n_node, occ_node * b->cu));
         sg.b_gather = std::max(KOR_NGROUPS, std::min((sg.n_gather + KOR_NGROUPS - 1) / KOR_NGROUPS * KOR_NGROUPS,
-                                                     2 * b->cu));
+                                                     occ_gather * b->cu));
         sg.b_apply = std::max(1, std::min(sg.n_apply, 4 * b->cu));
         GDT_CHECK(sg.node_items.ensure(sizeof(int4) * ni.size()) == 0 &&
                       sg.gather_items.ensure(sizeof(int4) * gi.size()) == 0 &&

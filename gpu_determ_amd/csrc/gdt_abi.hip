@@ -527,7 +527,7 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
         // gather tiles: per group, contiguous destination ranges of at most
         // KOR_TILE_DST destinations and about 1/NT of the group's entries; NT
         // is the largest count (<= KOR_TILES_MAX) for which every group needs at
-        // most KOR_TILES_MAX tiles (G * KOR_TILES_MAX blocks, two per CU)
+        // most KOR_TILES_MAX tiles (G * KOR_TILES_MAX blocks in one round)
         std::vector<int4> tiles;
         int ntile = 0, maxe = 0;
         for (int NT = KOR_TILES_MAX; NT >= 1; NT--) {

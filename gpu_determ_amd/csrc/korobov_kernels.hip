@@ -199,7 +199,8 @@ __device__ __forceinline__ void zero_empty(int ga, int gb, const int32_t *ptr, d
 //            their entries of the pass in entry order (fixed order:
 //            reproducible sums).
 // The destinations' row pointers and coordinates are loaded before phase 1.
-// Two blocks fit a CU (LDS), so the G * 64 blocks of a cell run in one round.
+// Tiles of about 3,000 entries (48 KB of LDS): the G * 96 blocks of a cell
+// run in one round, three per CU.
 #define TILE_THREADS 256
 #define TILE_WAVES (TILE_THREADS / WAVE)
 #define TILE_DPT (KOR_TILE_DST / TILE_THREADS)  // destinations per thread
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // All global loads of a thread (its node's partials, f and centre, and its
 // share of the moment sums) are issued together at the start, so the block
 // waits for memory once.
-#define APPLY_ROUNDS 2  // block sums per thread and moment with loads in flight (512 blocks: one round)
+#define APPLY_ROUNDS 3  // block sums per thread and moment with loads in flight (768 blocks: one round)
 #define APPLY_THREADS 256
 // Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
 // mp [20][nparts], fc [npts], Lp / Qp [G][npts], outputs [npts], X [5].
@@ -682,7 +683,7 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
 
 int kor_gather_parts(gdt_context *h) { return KOR_GROUPS * h->kntile; }
 
-// one block per tile (two per CU: the G * 64 tiles of a cell run in one round)
+// one block per tile (the G * KOR_TILES_MAX tiles of a cell run in one round)
 int kor_gather_blocks(gdt_context *h, int cells)
 {
     (void)cells;
