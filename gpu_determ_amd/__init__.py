@@ -44,7 +44,7 @@ ABI_SYMBOLS = (
     "gdt_gemm_nt", "gdt_split_f64", "gdt_matrix_mfma",
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
-    "gdt_init_mixture", "gdt_korobov_tiles",
+    "gdt_init_mixture", "gdt_korobov_tiles", "gdt_kmath_pow",
 )
 
 _lib = None
@@ -126,6 +126,7 @@ def lib():
         "gdt_surrogate_info": [vp, ip, ip, ip, C.POINTER(C.c_long)],
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
         "gdt_korobov_tiles": [vp, ip, ip, ip],
+        "gdt_kmath_pow": [vp, vp, vp, vp, C.c_long],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -260,6 +261,14 @@ class Context:
         g, t, cap = C.c_int(), C.c_int(), C.c_int()
         check(lib().gdt_korobov_tiles(self.h, C.byref(g), C.byref(t), C.byref(cap)))
         return {"groups": g.value, "tiles": t.value, "cap": cap.value}
+
+    def kmath_pow(self, x, r):
+        """x**r through the node terms' table-driven log / exp (FP64 device
+        arrays of equal shape)."""
+        assert x.dtype == np.float64 and r.dtype == np.float64 and x.shape == r.shape
+        out = DeviceArray(x.shape)
+        check(lib().gdt_kmath_pow(self.h, x.ptr, r.ptr, out.ptr, int(np.prod(x.shape))))
+        return out
 
     def korobov_export(self):
         info = self.korobov_info()

@@ -160,6 +160,7 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
             C.tiles = h->ktile.as<int4>();
             C.gptr = h->kgptr.as<int32_t>();
             C.ent = h->kgent.as<uint2>();
+            C.kmt = h->kmath.as<double>();
             C.xi = h->xi.as<double>();
             C.f = b->f[s];
             C.Qout = b->Q.empty() ? nullptr : b->Q[s];
@@ -231,7 +232,7 @@ int gdt_batch_create(gdt_batch_handle *out, int nclass, const gdt_handle *classe
         b->cls.push_back(h);
         b->f.push_back(f[s]);
         b->cells.push_back(cells[s]);
-        lds_node = std::max(lds_node, (int)sizeof(double) * h->n_pnt);
+        lds_node = std::max(lds_node, kor_node_lds(h));
         lds_gather = std::max(lds_gather, (int)sizeof(double2) * h->kcap);
     }
     b->device = classes[0]->device;

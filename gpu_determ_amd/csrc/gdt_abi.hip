@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "gdt_internal.h"
+#include "kmath.h"
 
 static thread_local std::string g_last_error;
 
@@ -68,6 +69,42 @@ KernelTimer::~KernelTimer()
         }                                                            \
         GDT_HIP(hipSetDevice((h)->device));                          \
     } while (0)
+
+// Tables of kmt_log / kmt_exp, in 80-bit long double arithmetic (64-bit
+// significand): every double entry and high + low pair is correct to well
+// below one FP64 ulp.
+void kmath_tables(double *T)
+{
+    // high parts with the low 20 significand bits cleared (33 bits), so that
+    // their products with exponents and multiples of ln2/64 are exact
+    auto high33 = [](long double v) {
+        double d = (double)v;
+        uint64_t u;
+        std::memcpy(&u, &d, 8);
+        u &= ~((uint64_t(1) << 20) - 1);
+        std::memcpy(&d, &u, 8);
+        return d;
+    };
+    for (int j = 0; j < KMT_LOG_N; j++) {
+        const double c = (double)(1.0L / (1.0L + ((long double)j + 0.5L) / 128.0L));
+        const long double v = -logl((long double)c);
+        T[KMT_C + j] = c;
+        T[KMT_LHI + j] = (double)v;
+        T[KMT_LLO + j] = (double)(v - (long double)(double)v);
+    }
+    for (int j = 0; j < KMT_EXP_N; j++) {
+        const long double v = exp2l((long double)j / 64.0L);
+        T[KMT_SHI + j] = (double)v;
+        T[KMT_SLO + j] = (double)(v - (long double)(double)v);
+    }
+    const long double ln2 = logl(2.0L);
+    T[KMT_K] = high33(ln2);
+    T[KMT_K + 1] = (double)(ln2 - (long double)T[KMT_K]);
+    T[KMT_K + 2] = (double)(64.0L / ln2);
+    T[KMT_K + 3] = high33(ln2 / 64.0L);
+    T[KMT_K + 4] = (double)(ln2 / 64.0L - (long double)T[KMT_K + 3]);
+    for (int j = KMT_K + 5; j < KMT_SIZE; j++) T[j] = 0.0;
+}
 
 extern "C" {
 
@@ -192,9 +229,12 @@ int gdt_create(gdt_handle *out, const gdt_grid_desc *gd, int device)
         h->h_xi[3 * v + 1] = gd->vmin[1] + h->d[1] * (2.0 * j + 1.0) / 2.0;
         h->h_xi[3 * v + 2] = gd->vmin[2] + h->d[2] * (2.0 * k + 1.0) / 2.0;
     }
+    std::vector<double> kmt(KMT_SIZE);
+    kmath_tables(kmt.data());
     int rc = GDT_OK;
     if (h->box2c.ensure(sizeof(int) * h->n_box) || h->cidx.ensure(sizeof(int) * 3 * cnt) ||
-        h->xi.ensure(sizeof(double) * 3 * cnt) || h->cmp2box.ensure(sizeof(int) * cnt)) {
+        h->xi.ensure(sizeof(double) * 3 * cnt) || h->cmp2box.ensure(sizeof(int) * cnt) ||
+        h->kmath.ensure(sizeof(double) * KMT_SIZE)) {
         gdt_set_error(GDT_E_NOMEM, "gdt_create: device allocation failed");
         rc = GDT_E_NOMEM;
     }
@@ -207,7 +247,8 @@ int gdt_create(gdt_handle *out, const gdt_grid_desc *gd, int device)
         (hipMemcpy(h->box2c.p, h->h_box2c.data(), sizeof(int) * h->n_box, hipMemcpyHostToDevice) != hipSuccess ||
          hipMemcpy(h->cidx.p, h->h_cidx.data(), sizeof(int) * 3 * cnt, hipMemcpyHostToDevice) != hipSuccess ||
          hipMemcpy(h->xi.p, h->h_xi.data(), sizeof(double) * 3 * cnt, hipMemcpyHostToDevice) != hipSuccess ||
-         hipMemcpy(h->cmp2box.p, cmp2box.data(), sizeof(int) * cnt, hipMemcpyHostToDevice) != hipSuccess)) {
+         hipMemcpy(h->cmp2box.p, cmp2box.data(), sizeof(int) * cnt, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(h->kmath.p, kmt.data(), sizeof(double) * KMT_SIZE, hipMemcpyHostToDevice) != hipSuccess)) {
         gdt_set_error(GDT_E_HIP, "gdt_create: upload failed");
         rc = GDT_E_HIP;
     }
@@ -225,9 +266,12 @@ int gdt_destroy(gdt_handle h)
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    DevBuf *bufs[] = {&h->box2c, &h->cidx, &h->xi, &h->cmp2box, &h->tb, &h->ta, &h->gptr, &h->gbox, &h->gval,
-                      &h->kidx, &h->kr, &h->kw, &h->knode, &h->kptr, &h->kent, &h->s_fbox, &h->s_out1, &h->s_out2,
-                      &h->s_vals, &h->s_loss, &h->s_Q, &h->s_nu, &h->s_tmp, &h->s_dt};
+    DevBuf *bufs[] = {&h->box2c, &h->cidx,   &h->xi,     &h->cmp2box, &h->tb,     &h->ta,    &h->gptr,  &h->gbox,
+                      &h->gval,  &h->kidx,   &h->kr,     &h->kw,      &h->knode,  &h->kptr,  &h->kent,  &h->kn4,
+                      &h->kn1,   &h->ke4,    &h->ke1,    &h->kwb,     &h->kgptr,  &h->kgent, &h->ktile, &h->kmath,
+                      &h->mx_bh, &h->mx_bl,  &h->mx_bf,  &h->mx_ah,   &h->mx_al,  &h->mx_af, &h->s_mx,  &h->s_fbox,
+                      &h->s_out1, &h->s_out2, &h->s_vals, &h->s_loss, &h->s_Q,   &h->s_nu,  &h->s_tmp, &h->s_dt,
+                      &h->s_mpart, &h->s_part};
     for (DevBuf *b : bufs) b->release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -954,6 +998,17 @@ int gdt_init_mixture(gdt_handle h, const double *params, int comps, long cells, 
     H_CHECK(h);
     GDT_CHECK(params && f && comps > 0 && cells > 0, GDT_E_ARG, "gdt_init_mixture: bad arguments");
     return launch_init_mixture(h, params, comps, cells, f);
+}
+
+int gdt_kmath_pow(gdt_handle h, const double *x, const double *r, double *out, long n)
+{
+    GDT_CHECK(h && x && r && out && n >= 0, GDT_E_ARG, "gdt_kmath_pow: bad arguments");
+    if (n == 0) return GDT_OK;
+    GDT_HIP(hipSetDevice(h->device));
+    int rc = launch_kmath_pow(h, x, r, out, n);
+    if (rc) return rc;
+    GDT_HIP(hipStreamSynchronize(h->stream));
+    return GDT_OK;
 }
 
 int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap)

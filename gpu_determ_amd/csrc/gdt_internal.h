@@ -74,6 +74,7 @@ struct gdt_context {
     DevBuf kn4, kn1, ke4, ke1, kwb;
     DevBuf kgptr, kgent;          // per node-group CSRs of the gather (uint2 entry records)
     DevBuf ktile;                 // gather tiles (d0, d1, e0, e1), [group][kntile]
+    DevBuf kmath;                 // log / exp tables of the node terms (KMT_SIZE doubles)
     int kntile = 0, kcap = 0;     // tiles per group, LDS entries per gather pass
     std::vector<int32_t> h_kptr;  // host copy of the CSR row pointers
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
@@ -107,6 +108,13 @@ struct KernelTimer {
     KernelTimer(gdt_context *h_, const char *name_);
     ~KernelTimer();
 };
+
+// Tables of kmt_log / kmt_exp (kmath.h), KMT_SIZE doubles, computed on the
+// host in extended precision.
+void kmath_tables(double *T);
+// Dynamic LDS bytes of the kernels that stage f (f + the tables).
+int kor_node_lds(const gdt_context *h);
+int launch_kmath_pow(gdt_context *h, const double *x, const double *r, double *out, long n);
 
 // ---- setup kernels (setup_kernels.hip) -----------------------------------
 int launch_build_tables(gdt_context *h, int mode);
@@ -143,6 +151,7 @@ struct KorClassDev {
     const int32_t *gptr;
     const uint2 *ent;
     const double *xi;
+    const double *kmt;  // log / exp tables
     double *f;     // [cells][npts], updated in place by steps
     double *Qout;  // [cells][npts] or NULL
     double2 *V;    // [slots][nn]

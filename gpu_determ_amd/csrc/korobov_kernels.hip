@@ -33,15 +33,17 @@
 
 #include "gdt_internal.h"
 #include "kernel_common.h"
+#include "kmath.h"
 
 #define KOR_THREADS 1024
 #define KOR_WAVES (KOR_THREADS / WAVE)
 #define KOR_GROUPS KOR_NGROUPS
 
-// D and 2 w P of one node from the LDS copy of f.  x = (a | b << 16,
-// l | m << 16, ls | ms << 16, num | den << 16), D2 in the low 12 bits of y.
-__device__ __forceinline__ void node_terms(const uint4 x, const uint32_t y, const double *fs, double K, double &D,
-                                           double &W2P, double &r)
+// D and 2 w P of one node from the LDS copy of f (log / exp tables at ts).
+// x = (a | b << 16, l | m << 16, ls | ms << 16, num | den << 16), D2 in the
+// low 12 bits of y.
+__device__ __forceinline__ void node_terms(const uint4 x, const uint32_t y, const double *fs, const double *ts,
+                                           double K, double &D, double &W2P, double &r)
 {
 #pragma clang fp contract(off)
     const int num = (int)(short)(x.w & 0xFFFFu), den = (int)(short)(x.w >> 16);
@@ -55,7 +57,7 @@ __device__ __forceinline__ void node_terms(const uint4 x, const uint32_t y, cons
         r = (double)num / (double)den;
         const double Ps = fs[x.z & 0xFFFFu] * fs[x.z >> 16];
         if (Pl > 0.0 && Ps > 0.0)
-            R = Pl * exp(r * log(Ps / Pl));
+            R = Pl * kmt_exp(r * kmt_log(Ps / Pl, ts), ts);
         else
             R = (1.0 - r) * Pl + r * Ps;
     }
@@ -64,11 +66,13 @@ __device__ __forceinline__ void node_terms(const uint4 x, const uint32_t y, cons
     W2P = 2.0 * (w * P);
 }
 
-// Stages f of cell c into LDS.  Every load of the thread is issued before the
+// Stages f of cell c into LDS fs[0, npts) and the log / exp tables after it,
+// fs[npts, npts + KMT_SIZE).  Every load of the thread is issued before the
 // first store, so the staging costs one memory round trip (npts <=
 // KOR_STAGE * KOR_THREADS = 19,456; the stencil build admits at most 19,200).
 #define KOR_STAGE 19
-__device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts)
+static_assert(KMT_SIZE <= KOR_THREADS, "one table entry per thread");
+__device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, const double *kmt)
 {
     double v[KOR_STAGE];
 #pragma unroll
@@ -76,11 +80,13 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts)
         const int t = j * KOR_THREADS + threadIdx.x;
         v[j] = fc[t < npts ? t : npts - 1];  // unconditional (clamped) loads: no branches
     }
+    const double tv = kmt[threadIdx.x < KMT_SIZE ? threadIdx.x : 0];
 #pragma unroll
     for (int j = 0; j < KOR_STAGE; j++) {
         const int t = j * KOR_THREADS + threadIdx.x;
         if (t < npts) fs[t] = v[j];
     }
+    if (threadIdx.x < KMT_SIZE) fs[npts + threadIdx.x] = tv;
 }
 
 // ---------------------------------------------------------------------------
@@ -96,7 +102,7 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts)
 // KOR_THREADS) of the cell whose f is fc; values to out (the cell's V).
 __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const uint4 *__restrict__ nrec,
                                            const uint32_t *__restrict__ nd2, double K, const double *__restrict__ fc,
-                                           double2 *__restrict__ out, int b, int B)
+                                           double2 *__restrict__ out, int b, int B, const double *__restrict__ kmt)
 {
 #pragma clang fp contract(off)
     const long stride = (long)B * KOR_THREADS;
@@ -111,11 +117,12 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
         xb = nrec[tb];
         yb = nd2[tb];
     }
-    stage_f(fs, fc, npts);
+    stage_f(fs, fc, npts, kmt);
     __syncthreads();
+    const double *ts = fs + npts;
     double Da, Wa, ra, Db, Wb, rb;
-    node_terms(xa, ya, fs, K, Da, Wa, ra);
-    node_terms(xb, yb, fs, K, Db, Wb, rb);
+    node_terms(xa, ya, fs, ts, K, Da, Wa, ra);
+    node_terms(xb, yb, fs, ts, K, Db, Wb, rb);
     // streaming (nontemporal) stores: the values are read by the next kernel
     // only, from other XCDs, so they need not linger in this XCD's L2
     if (ta < nn) {
@@ -128,7 +135,7 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
     }
     for (long t = tb + stride; t < nn; t += stride) {
         double D, W2P, r;
-        node_terms(nrec[t], nd2[t], fs, K, D, W2P, r);
+        node_terms(nrec[t], nd2[t], fs, ts, K, D, W2P, r);
         __builtin_nontemporal_store(D, &out[t].x);
         __builtin_nontemporal_store(W2P, &out[t].y);
     }
@@ -136,11 +143,12 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
 
 __global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, const uint4 *__restrict__ nrec,
                                                            const uint32_t *__restrict__ nd2, double K,
-                                                           const double *__restrict__ f, double2 *__restrict__ V)
+                                                           const double *__restrict__ f, double2 *__restrict__ V,
+                                                           const double *__restrict__ kmt)
 {
     extern __shared__ double fs[];
     const int c = blockIdx.y;
-    node_block(fs, npts, nn, nrec, nd2, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x);
+    node_block(fs, npts, nn, nrec, nd2, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x, kmt);
 }
 
 // ---------------------------------------------------------------------------
@@ -341,7 +349,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
                                                             const double *__restrict__ f,
                                                             const double *__restrict__ xi,
                                                             double *__restrict__ loss, double *__restrict__ Q,
-                                                            double *__restrict__ mpart)
+                                                            double *__restrict__ mpart, const double *__restrict__ kmt)
 {
 #pragma clang fp contract(off)
     extern __shared__ double fs[];
@@ -361,8 +369,9 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
         xb = e4[ea + WAVE + lane];
         yb = e1[ea + WAVE + lane];
     }
-    stage_f(fs, f + (size_t)c * npts, npts);
+    stage_f(fs, f + (size_t)c * npts, npts, kmt);
     __syncthreads();
+    const double *ts = fs + npts;
     double *Qc = Q + (size_t)c * npts;
     double *Lc = loss + (size_t)c * npts;
     double m[20];
@@ -387,7 +396,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
         int key = -2 - lane;
         if (valid) {
             double D, W2P, r;
-            node_terms(x, y, fs, K, D, W2P, r);
+            node_terms(x, y, fs, ts, K, D, W2P, r);
             const uint32_t role = (y >> 12) & 3u;
             if (role == 0u) {
                 q = -1.0 * D;
@@ -558,7 +567,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_batch_node(const KorClassDev *_
         const int4 it = items[i];
         const KorClassDev C = cls[it.x];
         node_block(fs, C.npts, C.nn, C.nrec, C.nd2, C.K, C.f + (size_t)it.y * C.npts, C.V + (size_t)it.w * C.nn,
-                   it.z, C.node_blocks);
+                   it.z, C.node_blocks, C.kmt);
     }
 }
 
@@ -629,6 +638,30 @@ int launch_batch_apply(hipStream_t s, bool correct, const KorClassDev *cls, cons
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// x^r = exp(r log x) through the node kernels' table-driven log and exp
+// (accuracy hook).
+__global__ __launch_bounds__(256) void k_kmath_pow(const double *__restrict__ x, const double *__restrict__ r,
+                                                   double *__restrict__ out, long n, const double *__restrict__ kmt)
+{
+    __shared__ double ts[KMT_SIZE];
+    for (int i = threadIdx.x; i < KMT_SIZE; i += blockDim.x) ts[i] = kmt[i];
+    __syncthreads();
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = kmt_exp(r[i] * kmt_log(x[i], ts), ts);
+}
+
+int launch_kmath_pow(gdt_context *h, const double *x, const double *r, double *out, long n)
+{
+    const long want = (n + 255) / 256;
+    const int blocks = (int)(want < 1024 ? (want < 1 ? 1 : want) : 1024);
+    hipLaunchKernelGGL(k_kmath_pow, dim3(blocks), dim3(256), 0, h->stream, x, r, out, n, h->kmath.as<double>());
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+// Dynamic LDS of the kernels that stage f: f and the log / exp tables.
+int kor_node_lds(const gdt_context *h) { return (int)(sizeof(double) * (h->n_pnt + KMT_SIZE)); }
+
 int kor_blocks(gdt_context *h, int cells)
 {
     (void)h;
@@ -660,7 +693,7 @@ static int kor_wave_bounds(gdt_context *h, int B)
 // Outside graph capture: LDS limits and the wave partition for `cells`.
 int kor_setup(gdt_context *h, int cells)
 {
-    const int lds = (int)(sizeof(double) * h->n_pnt);
+    const int lds = kor_node_lds(h);
     if (h->lds_attr < lds) {
         GDT_HIP(hipFuncSetAttribute((const void *)k_kor_node, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
         GDT_HIP(hipFuncSetAttribute((const void *)k_kor_fused, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
@@ -675,8 +708,8 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
 {
     const int B = kor_blocks(h, cells);
     KernelTimer kt(h, "kor_node");
-    hipLaunchKernelGGL(k_kor_node, dim3(B, cells), dim3(KOR_THREADS), sizeof(double) * h->n_pnt, h->stream, h->n_pnt,
-                       h->nn, h->kn4.as<uint4>(), h->kn1.as<uint32_t>(), h->kK, f, (double2 *)DW);
+    hipLaunchKernelGGL(k_kor_node, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->stream, h->n_pnt, h->nn,
+                       h->kn4.as<uint4>(), h->kn1.as<uint32_t>(), h->kK, f, (double2 *)DW, h->kmath.as<double>());
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
@@ -706,9 +739,9 @@ int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, d
     const int B = kor_blocks(h, cells);
     GDT_CHECK(h->wb_blocks == B, GDT_E_STATE, "fused: wave bounds for %d blocks not prepared", B);
     KernelTimer kt(h, "kor_fused");
-    hipLaunchKernelGGL(k_kor_fused, dim3(B, cells), dim3(KOR_THREADS), sizeof(double) * h->n_pnt, h->stream,
-                       h->n_pnt, h->kwb.as<int32_t>(), h->kptr.as<int32_t>(), h->ke4.as<uint4>(),
-                       h->ke1.as<uint32_t>(), h->kK, f, h->xi.as<double>(), loss, Q, mpart);
+    hipLaunchKernelGGL(k_kor_fused, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->stream, h->n_pnt,
+                       h->kwb.as<int32_t>(), h->kptr.as<int32_t>(), h->ke4.as<uint4>(), h->ke1.as<uint32_t>(), h->kK,
+                       f, h->xi.as<double>(), loss, Q, mpart, h->kmath.as<double>());
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }

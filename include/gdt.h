@@ -123,6 +123,10 @@ int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32
 /* Gather layout of the stencil: node groups (one per XCD), destination
  * tiles per group (one moment partial each), LDS entries per gather pass. */
 int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
+/* Accuracy hook, no reference counterpart: out[i] = x[i]^r[i] as the node
+ * terms compute it (table-driven FP64 log and exp); device pointers, x > 0;
+ * synchronous. */
+int gdt_kmath_pow(gdt_handle h, const double *x, const double *r, double *out, long n);
 /* Host copy of the CSR: ptr[n_pnt+1], ent[entries] = node*4 + role. */
 int gdt_korobov_export_csr(gdt_handle h, int32_t *ptr, uint32_t *ent);
 /* Generator choice alone (host): same rule as gdt_korobov_build. */

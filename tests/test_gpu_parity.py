@@ -309,3 +309,31 @@ def test_init_mixture_matches_numpy():
     f = ctx.init_mixture(par).get()
     ref = np.stack([sum(maxwellian(ctx.xi, *par[c, k, :2], par[c, k, 2:]) for k in range(comps)) for c in range(cells)])
     assert rel_l2(f, ref) < 1e-14
+
+
+def test_kmath_pow_accuracy():
+    """The node terms' table-driven FP64 log / exp (x^r = exp(r log x), x > 0,
+    r in [0, 1]) against numpy's pow.  Even exact log and exp carry the
+    rounding of y = r log x, a relative error of |y| 2^-53 in the result, so
+    the bound is 2^-52 (2 + 2 |y|): about two ulps of log and of exp."""
+    say("table-driven pow")
+    c = G.Context((8, 8, 8), (-1.0,) * 3, (1.0,) * 3, -1.0)
+    rng = np.random.default_rng(12)
+    n = 1 << 20
+    x = np.exp(rng.uniform(-690.0, 690.0, n))
+    x[: n // 4] = np.exp(rng.uniform(-2.0, 2.0, n // 4))  # ratios near 1, the common case
+    r = rng.uniform(0.0, 1.0, n)
+    special = [(1.0, 0.5), (0.5, 1.0), (2.0, 1.0), (1.0 - 2.0 ** -52, 0.3), (1.0 + 2.0 ** -52, 0.7),
+               (2.0 ** -1022, 0.5), (2.0 ** 1000, 0.25), (np.nextafter(2.0, 0.0), 1.0), (1.5, 0.0)]
+    for i, (a, b) in enumerate(special):
+        x[i], r[i] = a, b
+    got = c.kmath_pow(G.DeviceArray.from_host(x), G.DeviceArray.from_host(r)).get()
+    ref = x ** r
+    y = r * np.log(x)
+    err = np.abs(got - ref) / ref
+    ratio = err / (2.0 ** -52 * (2.0 + 2.0 * np.abs(y)))
+    say("max rel err %.3g, max err / bound %.3g" % (err.max(), ratio.max()))
+    assert np.all(np.isfinite(got))
+    assert ratio.max() <= 1.0, (x[ratio.argmax()], r[ratio.argmax()], err.max())
+    # special values to within two ulps: 1^0.5, 0.5^1, 2^1, 1.5^0
+    assert np.all(np.abs(got[[0, 1, 2, 8]] - [1.0, 0.5, 2.0, 1.0]) <= 2 * np.spacing([1.0, 0.5, 2.0, 1.0]))
