@@ -178,7 +178,7 @@ int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Q
 #define KOR_NGROUPS 8     // node groups of the gather: one per XCD
 #define KOR_TILE_DST 512  // destinations per gather tile: one per thread
 #define KOR_TILE_CAP 4864 // entries per gather pass (76 KB of LDS at most)
-#define KOR_TILES_MAX 96  // gather tiles per group: G * 96 blocks, three per CU, one round
+#define KOR_TILES_MAX 128 // gather tiles per group: G * 128 blocks, four per CU, one round
 int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells);
 // G group partials [cell][G][node] of Q and loss (G = 1: plain arrays)
 int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const double *mpart, const double *f,

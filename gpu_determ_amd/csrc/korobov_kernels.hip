@@ -207,15 +207,17 @@ __device__ __forceinline__ void zero_empty(int ga, int gb, const int32_t *ptr, d
 //            their entries of the pass in entry order (fixed order:
 //            reproducible sums).
 // The destinations' row pointers and coordinates are loaded before phase 1.
-// Tiles of about 3,000 entries (48 KB of LDS): the G * 96 blocks of a cell
-// run in one round, three per CU.
+// Tiles of about 2,300 entries (37 KB of LDS) and batches of TILE_BATCH
+// entries per thread (one batch per tile; 112 VGPRs): the G * 128 blocks of
+// a cell run in one round, four per CU.
 #define TILE_THREADS 256
 #define TILE_WAVES (TILE_THREADS / WAVE)
 #define TILE_DPT (KOR_TILE_DST / TILE_THREADS)  // destinations per thread
-#define TILE_BATCH 20
+#define TILE_BATCH 10
 
 // Block b (group b % G, tile b / G) of a cell: Vc its node values, Lp / Qp its
 // group partials [G][npts], mp its moment sums [20][nparts].
+template <int BATCH = TILE_BATCH>
 __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], int npts, int ntile, int cap,
                                              const int4 *__restrict__ tiles, const int32_t *__restrict__ gptr,
                                              const uint2 *__restrict__ ent, const double2 *__restrict__ Vc,
@@ -227,11 +229,17 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
     const int4 tl = tiles[(size_t)g * ntile + k];
     const int d0 = tl.x, d1 = tl.y, e0 = tl.z, e1 = tl.w;
     const int32_t *ptr = gptr + (size_t)g * (npts + 1);
+    // a tile of at most TILE_THREADS / 4 destinations (long rows) gives each
+    // destination a quad: lane r of the quad sums the entries r, r + 4, ...
+    // of the destination, the quad adds (s0 + s1) + (s2 + s3) -- the very
+    // sums of the one-thread form below, so both give the same bits
+    const bool quad = d1 - d0 <= TILE_THREADS / 4;
+    const bool owner = !quad || (threadIdx.x & 3) == 0;
     int ea[TILE_DPT], eb[TILE_DPT];
     double px[TILE_DPT], py[TILE_DPT], pz[TILE_DPT], q[TILE_DPT], l[TILE_DPT];
 #pragma unroll
     for (int i = 0; i < TILE_DPT; i++) {
-        const int d = d0 + threadIdx.x + i * TILE_THREADS;
+        const int d = quad ? (i == 0 ? d0 + (int)(threadIdx.x >> 2) : d1) : d0 + threadIdx.x + i * TILE_THREADS;
         ea[i] = eb[i] = 0;
         px[i] = py[i] = pz[i] = q[i] = l[i] = 0.0;
         if (d < d1) {
@@ -245,21 +253,21 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
     for (int w0 = e0; w0 < e1; w0 += cap) {
         const int w1 = min(e1, w0 + cap);
         if (w0 > e0) __syncthreads();  // the previous pass's reads are done
-        for (int j0 = w0 + threadIdx.x; j0 < w1; j0 += TILE_BATCH * TILE_THREADS) {
-            uint2 x[TILE_BATCH];
+        for (int j0 = w0 + threadIdx.x; j0 < w1; j0 += BATCH * TILE_THREADS) {
+            uint2 x[BATCH];
 #pragma unroll
-            for (int j = 0; j < TILE_BATCH; j++) {
+            for (int j = 0; j < BATCH; j++) {
                 const int e = j0 + j * TILE_THREADS;
                 x[j] = e < w1 ? ent[e] : make_uint2(0u, 0u);
             }
-            double2 v[TILE_BATCH];
+            double2 v[BATCH];
 #pragma unroll
-            for (int j = 0; j < TILE_BATCH; j++) {
+            for (int j = 0; j < BATCH; j++) {
                 const int e = j0 + j * TILE_THREADS;
                 v[j] = e < w1 ? Vc[x[j].x >> 2] : make_double2(0.0, 0.0);
             }
 #pragma unroll
-            for (int j = 0; j < TILE_BATCH; j++) {
+            for (int j = 0; j < BATCH; j++) {
                 const int e = j0 + j * TILE_THREADS;
                 if (e < w1) {
                     const uint32_t role = x[j].x & 3u;
@@ -271,6 +279,22 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
             }
         }
         __syncthreads();
+        if (quad) {
+            const int a = max(ea[0], w0), bb = min(eb[0], w1);
+            double qs = 0.0, ls = 0.0;
+            for (int e = a + (int)(threadIdx.x & 3); e < bb; e += 4) {
+                const double2 t = ql[e - w0];
+                qs += t.x;
+                ls += t.y;
+            }
+            qs += dpp_f64<DPP_QUAD_XOR1>(qs);
+            ls += dpp_f64<DPP_QUAD_XOR1>(ls);
+            qs += dpp_f64<DPP_QUAD_XOR2>(qs);
+            ls += dpp_f64<DPP_QUAD_XOR2>(ls);
+            q[0] += qs;
+            l[0] += ls;
+            continue;
+        }
         // four interleaved partial sums per destination (loads in flight
         // together), combined in a fixed order: reproducible
 #pragma unroll
@@ -302,8 +326,8 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
     for (int i = 0; i < 20; i++) m[i] = 0.0;
 #pragma unroll
     for (int i = 0; i < TILE_DPT; i++) {
-        const int d = d0 + threadIdx.x + i * TILE_THREADS;
-        if (d < d1) {
+        const int d = quad ? (i == 0 ? d0 + (int)(threadIdx.x >> 2) : d1) : d0 + threadIdx.x + i * TILE_THREADS;
+        if (d < d1 && owner) {
             const size_t po = (size_t)g * npts + d;
             __builtin_nontemporal_store(q[i], &Qp[po]);
             __builtin_nontemporal_store(l[i], &Lp[po]);
@@ -442,7 +466,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // All global loads of a thread (its node's partials, f and centre, and its
 // share of the moment sums) are issued together at the start, so the block
 // waits for memory once.
-#define APPLY_ROUNDS 3  // block sums per thread and moment with loads in flight (768 blocks: one round)
+#define APPLY_ROUNDS 4  // block sums per thread and moment with loads in flight (1024 blocks: one round)
 #define APPLY_THREADS 256
 // Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
 // mp [20][nparts], fc [npts], Lp / Qp [G][npts], outputs [npts], X [5].
