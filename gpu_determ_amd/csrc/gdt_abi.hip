@@ -44,21 +44,68 @@ void DevBuf::release()
     bytes = 0;
 }
 
+int timing_event(gdt_context *h)
+{
+    if (h->ev_used == (int)h->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1;
+        }
+        h->ev_pool.push_back(e);
+    }
+    return h->ev_used++;
+}
+
+int timing_resolve(gdt_context *h)
+{
+    if (!h->pending.empty()) {
+        GDT_HIP(hipEventSynchronize(h->ev_pool[h->pending.back().b]));
+        for (const auto &p : h->pending) {
+            float ms = 0.0f;
+            GDT_HIP(hipEventElapsedTime(&ms, h->ev_pool[p.a], h->ev_pool[p.b]));
+            KernelStat &st = h->stats[p.name];
+            st.ms += ms;
+            st.count += 1;
+        }
+        h->pending.clear();
+    }
+    h->ev_used = 0;
+    return GDT_OK;
+}
+
+ExtTimer::ExtTimer(gdt_context *h_, const char *name_) : h(h_), name(name_)
+{
+    if (!h->timing) return;
+    if (h->ev_used + 2 > 4096) timing_resolve(h);  // bounded pool: resolve, then reuse
+    a = timing_event(h);
+    b = a >= 0 ? timing_event(h) : -1;
+    if (a >= 0 && b >= 0) {
+        start = h->ev_pool[a];
+        stop = h->ev_pool[b];
+    }
+}
+
+ExtTimer::~ExtTimer()
+{
+    if (start) h->pending.push_back({name, a, b});
+}
+
 KernelTimer::KernelTimer(gdt_context *h_, const char *name_) : h(h_), name(name_)
 {
-    if (h->timing) (void)hipEventRecord(h->ev0, h->stream);
+    if (!h->timing) return;
+    if (h->ev_used + 2 > 4096) timing_resolve(h);  // bounded pool: resolve, then reuse
+    a = timing_event(h);
+    if (a >= 0) (void)hipEventRecord(h->ev_pool[a], h->stream);
 }
 
 KernelTimer::~KernelTimer()
 {
-    if (!h->timing) return;
-    (void)hipEventRecord(h->ev1, h->stream);
-    (void)hipEventSynchronize(h->ev1);
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
-    KernelStat &s = h->stats[name];
-    s.ms += ms;
-    s.count += 1;
+    if (!h->timing || a < 0) return;
+    const int b = timing_event(h);
+    if (b < 0) return;
+    (void)hipEventRecord(h->ev_pool[b], h->stream);
+    h->pending.push_back({name, a, b});
 }
 
 #define H_CHECK(h)                                                   \
@@ -275,6 +322,7 @@ int gdt_destroy(gdt_handle h)
     for (DevBuf *b : bufs) b->release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return GDT_OK;
@@ -971,6 +1019,8 @@ int gdt_kernel_stats(gdt_handle h, const char *name, double *total_ms, long *cou
 {
     H_CHECK(h);
     GDT_CHECK(name, GDT_E_ARG, "gdt_kernel_stats: name is null");
+    int rc = timing_resolve(h);
+    if (rc) return rc;
     auto it = h->stats.find(name);
     if (total_ms) *total_ms = it == h->stats.end() ? 0.0 : it->second.ms;
     if (count) *count = it == h->stats.end() ? 0 : it->second.count;
@@ -980,8 +1030,9 @@ int gdt_kernel_stats(gdt_handle h, const char *name, double *total_ms, long *cou
 int gdt_kernel_stats_reset(gdt_handle h)
 {
     H_CHECK(h);
+    int rc = timing_resolve(h);
     h->stats.clear();
-    return GDT_OK;
+    return rc;
 }
 
 int gdt_transpose(gdt_handle h, const double *src, double *dst, int rows, int cols)

@@ -1,6 +1,7 @@
 // Internal state and kernel launchers of libgdt.so (not installed).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -99,12 +100,22 @@ struct gdt_context {
     bool timing = false;
     std::map<std::string, KernelStat> stats;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // timed launches: events recorded before and after each, back to back on
+    // the stream (no wait between launches), resolved when the stats are read
+    struct PendingTime {
+        const char *name;
+        int a, b;
+    };
+    std::vector<hipEvent_t> ev_pool;
+    int ev_used = 0;
+    std::vector<PendingTime> pending;
 };
 
 // Records a kernel launch with hipEvents on the context stream when timing is on.
 struct KernelTimer {
     gdt_context *h;
     const char *name;
+    int a = -1;
     KernelTimer(gdt_context *h_, const char *name_);
     ~KernelTimer();
 };
@@ -115,6 +126,34 @@ void kmath_tables(double *T);
 // Dynamic LDS bytes of the kernels that stage f (f + the tables).
 int kor_node_lds(const gdt_context *h);
 int launch_kmath_pow(gdt_context *h, const double *x, const double *r, double *out, long n);
+
+// Waits for the pending timed launches and adds them to the stats.
+int timing_resolve(gdt_context *h);
+// Index of a free event of the context's timing pool (-1 if none can be made).
+int timing_event(gdt_context *h);
+
+// Start / stop events of a timed launch of the hot-path kernels: passed to
+// hipExtLaunchKernelGGL, which records them in the kernel's own dispatch, so
+// the elapsed time is the kernel's execution (no extra packets around it).
+struct ExtTimer {
+    gdt_context *h;
+    const char *name;
+    int a = -1, b = -1;
+    hipEvent_t start = nullptr, stop = nullptr;
+    ExtTimer(gdt_context *h_, const char *name_);
+    ~ExtTimer();
+};
+
+// Launches `kern` on the context stream, through hipExtLaunchKernelGGL with
+// the ExtTimer events when timing is on.
+#define GDT_LAUNCH_TIMED(h, name, kern, grid, block, lds, ...)                                                  \
+    do {                                                                                                       \
+        ExtTimer et_((h), (name));                                                                             \
+        if (et_.start)                                                                                         \
+            hipExtLaunchKernelGGL(kern, grid, block, lds, (h)->stream, et_.start, et_.stop, 0, __VA_ARGS__);  \
+        else                                                                                                   \
+            hipLaunchKernelGGL(kern, grid, block, lds, (h)->stream, __VA_ARGS__);                              \
+    } while (0)
 
 // ---- setup kernels (setup_kernels.hip) -----------------------------------
 int launch_build_tables(gdt_context *h, int mode);

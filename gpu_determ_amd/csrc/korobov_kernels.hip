@@ -731,9 +731,8 @@ int kor_setup(gdt_context *h, int cells)
 int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
 {
     const int B = kor_blocks(h, cells);
-    KernelTimer kt(h, "kor_node");
-    hipLaunchKernelGGL(k_kor_node, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->stream, h->n_pnt, h->nn,
-                       h->kn4.as<uint4>(), h->kn1.as<uint32_t>(), h->kK, f, (double2 *)DW, h->kmath.as<double>());
+    GDT_LAUNCH_TIMED(h, "kor_node", k_kor_node, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->n_pnt, h->nn,
+                     h->kn4.as<uint4>(), h->kn1.as<uint32_t>(), h->kK, f, (double2 *)DW, h->kmath.as<double>());
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
@@ -750,10 +749,10 @@ int kor_gather_blocks(gdt_context *h, int cells)
 int launch_kor_gather(gdt_context *h, const double *V, double *Lpart, double *Qpart, double *mpart, int cells)
 {
     const int B = kor_gather_blocks(h, cells);
-    KernelTimer kt(h, "kor_gather");
-    hipLaunchKernelGGL(k_kor_gather, dim3(B, cells), dim3(TILE_THREADS), sizeof(double2) * h->kcap, h->stream,
-                       h->n_pnt, h->nn, h->kntile, h->kcap, h->ktile.as<int4>(), h->kgptr.as<int32_t>(),
-                       h->kgent.as<uint2>(), (const double2 *)V, h->xi.as<double>(), Lpart, Qpart, mpart);
+    GDT_LAUNCH_TIMED(h, "kor_gather", k_kor_gather, dim3(B, cells), dim3(TILE_THREADS),
+                     (uint32_t)(sizeof(double2) * h->kcap), h->n_pnt, h->nn, h->kntile, h->kcap,
+                     h->ktile.as<int4>(), h->kgptr.as<int32_t>(), h->kgent.as<uint2>(), (const double2 *)V,
+                     h->xi.as<double>(), Lpart, Qpart, mpart);
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
@@ -762,10 +761,9 @@ int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, d
 {
     const int B = kor_blocks(h, cells);
     GDT_CHECK(h->wb_blocks == B, GDT_E_STATE, "fused: wave bounds for %d blocks not prepared", B);
-    KernelTimer kt(h, "kor_fused");
-    hipLaunchKernelGGL(k_kor_fused, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->stream, h->n_pnt,
-                       h->kwb.as<int32_t>(), h->kptr.as<int32_t>(), h->ke4.as<uint4>(), h->ke1.as<uint32_t>(), h->kK,
-                       f, h->xi.as<double>(), loss, Q, mpart, h->kmath.as<double>());
+    GDT_LAUNCH_TIMED(h, "kor_fused", k_kor_fused, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->n_pnt,
+                     h->kwb.as<int32_t>(), h->kptr.as<int32_t>(), h->ke4.as<uint4>(), h->ke1.as<uint32_t>(), h->kK,
+                     f, h->xi.as<double>(), loss, Q, mpart, h->kmath.as<double>());
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
@@ -776,11 +774,10 @@ int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const doub
 {
     GDT_CHECK(G == 1 || G == KOR_GROUPS, GDT_E_ARG, "kor_apply: %d partials per node", G);
     const dim3 grid((h->n_pnt + 255) / 256, cells);
-    KernelTimer kt(h, "kor_apply");
     auto kern = correct ? (G == 1 ? k_kor_apply<true, 1> : k_kor_apply<true, KOR_GROUPS>)
                         : (G == 1 ? k_kor_apply<false, 1> : k_kor_apply<false, KOR_GROUPS>);
-    hipLaunchKernelGGL(kern, grid, dim3(256), 0, h->stream, h->n_pnt, nparts, mpart, h->xi.as<double>(), f, Lpart,
-                       Qpart, Qout, Lout, nu_out, X, dt, f_upd);
+    GDT_LAUNCH_TIMED(h, "kor_apply", kern, grid, dim3(256), 0u, h->n_pnt, nparts, mpart, h->xi.as<double>(), f, Lpart,
+                     Qpart, Qout, Lout, nu_out, X, dt, f_upd);
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }

@@ -251,7 +251,9 @@ int gdt_init_mixture(gdt_handle h, const double *params, int comps, long cells, 
 
 /* ---- measurement --------------------------------------------------------- */
 /* Kernel timing under GDT_FLAG_TIME: total milliseconds and launch count of
- * the kernel `name` since the last reset. */
+ * the kernel `name` since the last reset.  Each timed launch is bracketed by
+ * two events on the context stream, back to back with the other launches (no
+ * wait in between); reading the stats waits for them. */
 int gdt_kernel_stats(gdt_handle h, const char *name, double *total_ms, long *count);
 int gdt_kernel_stats_reset(gdt_handle h);
 /* Layout transposes (transpose_float, collision_integral.cu:528-547):
