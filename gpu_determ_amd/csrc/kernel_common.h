@@ -161,10 +161,22 @@ __device__ __forceinline__ bool solve5(double (&M)[5][6], double (&X)[5])
     return ok;
 }
 
+// 1 / sqrt(d) for d > 0 to FP64 accuracy: the hardware estimate and two
+// Newton steps y <- y (3/2 - d y^2 / 2).
+__device__ __forceinline__ double rsqrt_nr(double d)
+{
+    const double h = 0.5 * d;
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
 // Cholesky solve of the symmetric positive definite 5x5 system (moment
 // matrices sum L psi psi^T with L >= 0): short dependency chains, registers
-// only.  A non-positive pivot (not positive definite, e.g. negative f) falls
-// back to the pivoted elimination.
+// only, the pivots' inverse square roots by rsqrt_nr (no division or square
+// root on the chain).  A non-positive pivot (not positive definite, e.g.
+// negative f) falls back to the pivoted elimination.
 __device__ __forceinline__ void solve5_spd(const double (&A)[5][5], const double (&B)[5], double (&X)[5])
 {
     double Lm[5][5], inv[5];
@@ -175,9 +187,7 @@ __device__ __forceinline__ void solve5_spd(const double (&A)[5][5], const double
 #pragma unroll
         for (int k = 0; k < j; k++) d -= Lm[j][k] * Lm[j][k];
         spd = spd && (d > 0.0);
-        const double ljj = sqrt(d > 0.0 ? d : 1.0);
-        inv[j] = 1.0 / ljj;
-        Lm[j][j] = ljj;
+        inv[j] = rsqrt_nr(d > 0.0 ? d : 1.0);
 #pragma unroll
         for (int i = j + 1; i < 5; i++) {
             double v = A[i][j];
