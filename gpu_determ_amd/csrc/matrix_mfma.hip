@@ -14,13 +14,14 @@
 // A > 15 or s - n < 0 on an axis, :88-131).  Per parity class of s, G is a
 // GEMM: (512 g x 512 n1) times (512 n1 x S_class s).  A block owns one cell
 // and one class: f in LDS (FP64), the a operand streamed in K slices, T built
-// on the fly for a tile of 128 s, 8 waves of 64 g x 128 s MFMA tiles; every
-// G element is then added to gain[(s + sigma g) / 2] for each sign pattern
-// sigma with both velocities on the grid (the pairs (v, v1) of the
-// reference), in an FP32 LDS copy of the cell's gain.  The block writes its
-// class's partial gain; the 8 partials are summed in FP64.  The LDS adds
-// come in no fixed order (FP32 rounding differences of ~1e-7 relative run to
-// run, far below the bf16 operand rounding this path is stated with).
+// on the fly for a tile of 128 s (one ix, eight iy rows of 16 iz, padded where
+// the class has 15), 8 waves of 64 g x 128 s MFMA tiles.  The tile's G is then
+// read back by output: thread t owns the outputs v = t + 512 k (k < 8) and,
+// chunk by chunk (two iy rows of the tile staged in LDS), adds G[|2v - s|, s]
+// for every column s whose partner v1 = s - v is on the grid (the pairs
+// (v, v1) of the reference with v + v1 = s), in FP64 registers and in a fixed
+// order: no atomics, bitwise reproducible.  The block writes its class's
+// partial gain; the 8 partials are summed in FP64.
 //
 // Operand precision (mode): bf16, bf16 split into hi + lo (C = Ah Bh + Ah Bl
 // + Al Bh), or fp32 MFMA.  The conservation correction and Q = gain - f nu
@@ -41,6 +42,17 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 #define MX_KF 16          // fp32 K slice
 #define MX_PB 40          // bf16 LDS row pitch (elements)
 #define MX_PF 17          // fp32 LDS row pitch (elements)
+#define MX_GP 33          // LDS row pitch of a staged G chunk (floats)
+
+// Bytes of the operand region: the operand tiles of the mode, or a staged G
+// chunk (512 rows x 32 columns), whichever is larger.
+__host__ __device__ constexpr size_t mx_ops_bytes(int mode)
+{
+    const size_t opnd = mode == 2 ? sizeof(float) * (MX_G + MX_S) * MX_PF
+                                  : sizeof(uint16_t) * (mode == 1 ? 2 : 1) * (MX_G + MX_S) * MX_PB;
+    const size_t gch = sizeof(float) * MX_G * MX_GP;
+    return opnd > gch ? opnd : gch;
+}
 
 __device__ __forceinline__ uint16_t mx_bf16(float x)
 {
@@ -49,20 +61,15 @@ __device__ __forceinline__ uint16_t mx_bf16(float x)
 }
 __device__ __forceinline__ float mx_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// s index within class p (axis parity p_a): s_a = 2 i_a + p_a, i_a < 16 - p_a.
-__device__ __forceinline__ void mx_s_of(int p, int si, int &sx, int &sy, int &sz)
+// Column c of tile t of class p: ix = t / 2, iy = 8 (t % 2) + c / 16,
+// iz = c % 16, s_a = 2 i_a + p_a; -1 when off the class (i_a >= 16 - p_a),
+// else (sx, sy, sz) packed by bytes.
+__device__ __forceinline__ int mx_col(int p, int t, int c)
 {
     const int px = (p >> 2) & 1, py = (p >> 1) & 1, pz = p & 1;
-    const int nz = MX_N - pz, ny = MX_N - py;
-    const int iz = si % nz, iy = (si / nz) % ny, ix = si / (nz * ny);
-    sx = 2 * ix + px;
-    sy = 2 * iy + py;
-    sz = 2 * iz + pz;
-}
-
-__device__ __forceinline__ int mx_s_count(int p)
-{
-    return (MX_N - ((p >> 2) & 1)) * (MX_N - ((p >> 1) & 1)) * (MX_N - (p & 1));
+    const int ix = t >> 1, iy = 8 * (t & 1) + (c >> 4), iz = c & 15;
+    if (ix >= MX_N - px || iy >= MX_N - py || iz >= MX_N - pz) return -1;
+    return (2 * ix + px) | ((2 * iy + py) << 8) | ((2 * iz + pz) << 16);
 }
 
 // T[n1, s] from the LDS copy of f (FP64), with the reference's drop rule.
@@ -80,26 +87,6 @@ __device__ __forceinline__ double mx_T(const double *fs, int p, int sx, int sy, 
 #undef MF
 }
 
-// Adds G[g, s] (g = class row gr) to every output v = (s + sigma g) / 2 whose
-// partner v1 = (s - sigma g) / 2 is on the grid too.  Per axis both sign
-// choices are on the grid exactly when s >= g and (s + g) / 2 <= 15 (one
-// choice when g = 0), so validity is one test per element.
-__device__ __forceinline__ void mx_scatter(float *gl, int p, int gr, int sx, int sy, int sz, float val)
-{
-    const int gx = 2 * (gr >> 6) + ((p >> 2) & 1), gy = 2 * ((gr >> 3) & 7) + ((p >> 1) & 1),
-              gz = 2 * (gr & 7) + (p & 1);
-    if (sx < gx || sy < gy || sz < gz || sx + gx > 2 * (MX_N - 1) || sy + gy > 2 * (MX_N - 1) ||
-        sz + gz > 2 * (MX_N - 1))
-        return;
-    const int hx = (sx + gx) >> 1, lx = (sx - gx) >> 1, hy = (sy + gy) >> 1, ly = (sy - gy) >> 1,
-              hz = (sz + gz) >> 1, lz = (sz - gz) >> 1;
-    const int nx = gx ? 2 : 1, ny = gy ? 2 : 1, nz = gz ? 2 : 1;
-    for (int a = 0; a < nx; a++)
-        for (int b = 0; b < ny; b++)
-            for (int c = 0; c < nz; c++)
-                atomicAdd(&gl[((a ? lx : hx) * MX_N + (b ? ly : hy)) * MX_N + (c ? lz : hz)], val);
-}
-
 // MODE: 0 bf16, 1 bf16 split, 2 fp32.  Grid (8 classes, cells); AwT[p][g][n1]
 // (hi, lo for split; fp32 in Af for MODE 2); out gpart[c][p][v] (FP32).
 template <int MODE>
@@ -110,23 +97,34 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
     extern __shared__ __align__(16) unsigned char mx_lds[];
     constexpr int NP = MODE == 1 ? 2 : 1;
     double *fs = (double *)mx_lds;                       // 32 KB
-    float *gl = (float *)(mx_lds + 8 * MX_NB);           // 16 KB
-    unsigned char *ops = mx_lds + 12 * MX_NB;            // operand tiles
+    unsigned char *ops = mx_lds + 8 * MX_NB;             // operand tiles, then a G chunk
     uint16_t *sA = (uint16_t *)ops;                      // [NP][512][MX_PB]
     uint16_t *sB = sA + NP * MX_G * MX_PB;               // [NP][128][MX_PB]
     float *fA = (float *)ops;                            // [512][MX_PF]
     float *fB = fA + MX_G * MX_PF;                       // [128][MX_PF]
+    float *Gc = (float *)ops;                            // [512][MX_GP]: 32 columns of G
+    int *scol = (int *)(ops + mx_ops_bytes(MODE));       // the tile's columns (mx_col)
     const int p = blockIdx.x, c = blockIdx.y;
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     const double *fc = f + (size_t)c * MX_NB;
-    for (int i = threadIdx.x; i < MX_NB; i += MX_T) {
-        fs[i] = fc[i];
-        gl[i] = 0.0f;
-    }
-    __syncthreads();
-    const int S = mx_s_count(p);
+    for (int i = threadIdx.x; i < MX_NB; i += MX_T) fs[i] = fc[i];
+    const int px = (p >> 2) & 1, py = (p >> 1) & 1, pz = p & 1;
+    const int NT = 2 * (MX_N - px);  // tiles: one ix, eight iy rows each
     const size_t aoff = (size_t)p * MX_G * MX_K;
-    for (int s0 = 0; s0 < S; s0 += MX_S) {
+    // outputs v = threadIdx.x + 512 k: vx = 2 k + w / 4 (the same in a wave),
+    // vy = (threadIdx.x / 16) % 16, vz = threadIdx.x % 16
+    const int vy = (threadIdx.x >> 4) & 15, vz = threadIdx.x & 15;
+    double gacc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) gacc[k] = 0.0;
+    // the iz of the outputs' partners v1 = s - v on the grid: sz - vz in
+    // [0, 15], i.e. iz in [izlo, izlo + 7] (at most 8 values of step 2)
+    const int izlo = (vz - pz + 1) >> 1;
+    for (int t = 0; t < NT; t++) {
+        // the tile's column coordinates, once (the K loop's first barrier
+        // publishes them); the previous tile's reads are done first
+        __syncthreads();
+        if (threadIdx.x < MX_S) scol[threadIdx.x] = mx_col(p, t, threadIdx.x);
         f32x16 acc[2][4];
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -153,15 +151,12 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
 #pragma unroll 2
                 for (int i = 0; i < 8; i++) {
                     const int q = threadIdx.x + i * MX_T, sc = q / KB, nk = q % KB;
-                    double t = 0.0;
-                    if (s0 + sc < S) {
-                        int sx, sy, sz;
-                        mx_s_of(p, s0 + sc, sx, sy, sz);
-                        t = mx_T(fs, p, sx, sy, sz, k0 + nk);
-                    }
-                    const uint16_t h = mx_bf16((float)t);
+                    double tv = 0.0;
+                    const int pk = scol[sc];
+                    if (pk >= 0) tv = mx_T(fs, p, pk & 255, (pk >> 8) & 255, pk >> 16, k0 + nk);
+                    const uint16_t h = mx_bf16((float)tv);
                     sB[sc * MX_PB + nk] = h;
-                    if (NP == 2) sB[(MX_S + sc) * MX_PB + nk] = mx_bf16((float)(t - (double)mx_f32(h)));
+                    if (NP == 2) sB[(MX_S + sc) * MX_PB + nk] = mx_bf16((float)(tv - (double)mx_f32(h)));
                 }
             } else {
 #pragma unroll
@@ -177,13 +172,10 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
 #pragma unroll 2
                 for (int i = 0; i < 4; i++) {
                     const int q = threadIdx.x + i * MX_T, sc = q / KB, nk = q % KB;
-                    double t = 0.0;
-                    if (s0 + sc < S) {
-                        int sx, sy, sz;
-                        mx_s_of(p, s0 + sc, sx, sy, sz);
-                        t = mx_T(fs, p, sx, sy, sz, k0 + nk);
-                    }
-                    fB[sc * MX_PF + nk] = (float)t;
+                    double tv = 0.0;
+                    const int pk = scol[sc];
+                    if (pk >= 0) tv = mx_T(fs, p, pk & 255, (pk >> 8) & 255, pk >> 16, k0 + nk);
+                    fB[sc * MX_PF + nk] = (float)tv;
                 }
             }
             __syncthreads();
@@ -230,33 +222,45 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
                 }
             }
         }
-        // scatter the tile's G into the cell's gain: each 32 x 32 MFMA tile is
-        // staged in the wave's LDS slot (operand space, free after the K loop),
-        // then walked by a small runtime loop
-        float *stg = (float *)ops + w * 32 * 33;
+        // read the tile's G back by output, 32 columns (two iy rows) at a
+        // time: stage the chunk in LDS (operand space, free after the K loop)
+        const int sx = 2 * (t >> 1) + px;
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 4; j++) {  // unrolled: acc[i][j] stays in registers
+            __syncthreads();  // operand reads (j = 0) / the previous chunk's reads are done
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                __syncthreads();  // operand reads / previous staging reads done
+            for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int e = 0; e < 16; e++) stg[((e / 4) * 8 + (l / 32) * 4 + e % 4) * 33 + l % 32] = acc[i][j][e];
-                __syncthreads();
-                for (int e = 0; e < 16; e++) {
-                    const int q = l + e * WAVE, r = q / 32, cc = q % 32;
-                    const int sc = s0 + j * 32 + cc;
-                    const float val = stg[r * 33 + cc];
-                    if (sc < S && val != 0.0f) {
-                        int sx, sy, sz;
-                        mx_s_of(p, sc, sx, sy, sz);
-                        mx_scatter(gl, p, w * 64 + i * 32 + r, sx, sy, sz, val);
+                for (int e = 0; e < 16; e++)
+                    Gc[(w * 64 + i * 32 + (e / 4) * 8 + (l / 32) * 4 + e % 4) * MX_GP + l % 32] = acc[i][j][e];
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int iy = 8 * (t & 1) + 2 * j + a;
+                if (iy >= MX_N - py) continue;
+                const int sy = 2 * iy + py;
+                if ((unsigned)(sy - vy) > 15u) continue;
+                const int gy = abs(2 * vy - sy) >> 1;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int vx = 2 * k + (w >> 2);  // the same in the wave
+                    if ((unsigned)(sx - vx) > 15u) continue;
+                    const int base = (64 * (abs(2 * vx - sx) >> 1) + 8 * gy) * MX_GP + 16 * a;
+                    double sum = gacc[k];
+#pragma unroll
+                    for (int m = 0; m < 8; m++) {
+                        const int iz = izlo + m, sz = 2 * iz + pz;
+                        if (iz < MX_N - pz && sz - vz <= 15)
+                            sum += (double)Gc[base + (abs(2 * vz - sz) >> 1) * MX_GP + iz];
                     }
+                    gacc[k] = sum;
                 }
             }
+        }
     }
-    __syncthreads();
     float *out = gpart + ((size_t)c * 8 + p) * MX_NB;
-    for (int i = threadIdx.x; i < MX_NB; i += MX_T) out[i] = gl[i];
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[threadIdx.x + MX_T * k] = (float)gacc[k];
 }
 
 // Operands from the FP64 tables: B[k][j] = b(|k - j|) (hi, lo, fp32) and
@@ -318,12 +322,11 @@ int launch_mx_operands(hipStream_t s, const double *b, const double *a, uint16_t
     return GDT_OK;
 }
 
+// f (FP64), the operand region, the tile's column coordinates.
 static size_t mx_lds_bytes(int mode)
 {
-    const size_t base = 12 * MX_NB;
-    if (mode == GDT_GEMM_F32) return base + sizeof(float) * (MX_G + MX_S) * MX_PF;
-    const int np = mode == GDT_GEMM_BF16_SPLIT ? 2 : 1;
-    return base + sizeof(uint16_t) * np * (MX_G + MX_S) * MX_PB;
+    const int m = mode == GDT_GEMM_F32 ? 2 : (mode == GDT_GEMM_BF16_SPLIT ? 1 : 0);
+    return 8 * MX_NB + mx_ops_bytes(m) + sizeof(int) * MX_S;
 }
 
 int launch_mx_gain(hipStream_t s, int mode, const uint16_t *Ah, const uint16_t *Al, const float *Af, const double *f,

@@ -93,3 +93,17 @@ def test_matrix_mfma_vs_oracle(box16):
     nu, gain, _ = ctx.matrix_mfma(G.GEMM_BF16_SPLIT, G.DeviceArray.from_host(f), 1, 0)
     assert rel_l2(nu.get(), onu) < 1e-4
     assert rel_l2(gain.get(), og) < 1e-4
+
+
+def test_matrix_mfma_bitwise_reproducible(box16):
+    """The gain's read-back is a per-output register sum in a fixed order (no
+    atomics), so two runs give the same bits in every operand mode."""
+    say("matrix mfma reproducibility")
+    ctx = box16
+    rng = np.random.default_rng(6)
+    fd = G.DeviceArray.from_host(perturbed(ctx.xi, 3, rng))
+    for mode in (G.GEMM_BF16, G.GEMM_BF16_SPLIT, G.GEMM_F32):
+        a = [x.get() for x in ctx.matrix_mfma(mode, fd, 3, G.FLAG_CORRECT)]
+        b = [x.get() for x in ctx.matrix_mfma(mode, fd, 3, G.FLAG_CORRECT)]
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
