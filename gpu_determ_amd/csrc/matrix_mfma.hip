@@ -121,6 +121,7 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
     // [0, 15], i.e. iz in [izlo, izlo + 7] (at most 8 values of step 2)
     const int izlo = (vz - pz + 1) >> 1;
     for (int t = 0; t < NT; t++) {
+        const int sx_t = 2 * (t >> 1) + px;
         // the tile's column coordinates, once (the K loop's first barrier
         // publishes them); the previous tile's reads are done first
         __syncthreads();
@@ -147,16 +148,34 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
                             *(const uint4 *)(src + aoff + (size_t)row * MX_K + k0 + col);
                     }
                 }
-                // T slice: 128 s x 32 n1, 8 per thread
-#pragma unroll 2
-                for (int i = 0; i < 8; i++) {
-                    const int q = threadIdx.x + i * MX_T, sc = q / KB, nk = q % KB;
-                    double tv = 0.0;
-                    const int pk = scol[sc];
-                    if (pk >= 0) tv = mx_T(fs, p, pk & 255, (pk >> 8) & 255, pk >> 16, k0 + nk);
-                    const uint16_t h = mx_bf16((float)tv);
-                    sB[sc * MX_PB + nk] = h;
-                    if (NP == 2) sB[(MX_S + sc) * MX_PB + nk] = mx_bf16((float)(tv - (double)mx_f32(h)));
+                // T slice: 128 s x 32 n1, 8 per thread.  Thread q holds n1 =
+                // k0 + q % 32 and the column iz = q / 32 of the tile's eight iy
+                // rows: n1, sx and sz (the x and z halves of A = (s + n) / 2
+                // and B = (s - n) / 2) are fixed; row by row sy grows by 2, so
+                // Ay and By step by one.
+                {
+                    const int nk = threadIdx.x % KB, iz = threadIdx.x / KB, n1 = k0 + nk;
+                    const int nx = 2 * (n1 >> 6) + px, ny = 2 * ((n1 >> 3) & 7) + py, nz = 2 * (n1 & 7) + pz;
+                    const int sz = 2 * iz + pz;
+                    const int ax = (sx_t + nx) >> 1, mx = sx_t - nx, az = (sz + nz) >> 1, mz = sz - nz;
+                    const bool okxz = iz < MX_N - pz && ax <= MX_N - 1 && mx >= 0 && az <= MX_N - 1 && mz >= 0;
+                    const int xa = ax * MX_N * MX_N, xb = (mx >> 1) * MX_N * MX_N, za = az, zb = mz >> 1;
+                    const int iy0 = 8 * (t & 1);
+#pragma unroll 1
+                    for (int i = 0; i < 8; i++) {
+                        const int iy = iy0 + i, sy = 2 * iy + py;
+                        const int ay = (sy + ny) >> 1, my = sy - ny;
+                        double tv = 0.0;
+                        if (okxz && iy < MX_N - py && ay <= MX_N - 1 && my >= 0) {
+                            const int ya = ay * MX_N, yb = (my >> 1) * MX_N;
+                            tv = fs[xa + ya + za] * fs[xb + yb + zb] + fs[xa + yb + zb] * fs[xb + ya + za] +
+                                 fs[xa + ya + zb] * fs[xb + yb + za] + fs[xa + yb + za] * fs[xb + ya + zb];
+                        }
+                        const int sc = 16 * i + iz;
+                        const uint16_t h = mx_bf16((float)tv);
+                        sB[sc * MX_PB + nk] = h;
+                        if (NP == 2) sB[(MX_S + sc) * MX_PB + nk] = mx_bf16((float)(tv - (double)mx_f32(h)));
+                    }
                 }
             } else {
 #pragma unroll
