@@ -1,7 +1,7 @@
 // Uneven batches of the Korobov method (gdt_batch_* in include/gdt.h): many
 // cells on several velocity grids, one corrected evaluation (or explicit
-// step) of every cell per run step, as three persistent work-list kernels per
-// segment of cells, the whole step captured as one hipGraph.
+// step) of every cell per run step, as three work-list kernels per segment of
+// cells, the whole step captured as one hipGraph.
 //
 // A segment holds up to seg_cells cells (of any grids, in class order); its
 // scratch (node values, group partials, moment sums) is reused by the next
@@ -127,15 +127,15 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
         sg.n_node = (int)ni.size();
         sg.n_gather = (int)gi.size();
         sg.n_apply = (int)ai.size();
-        // persistent grids: node blocks hold f in LDS (one or two per CU), gather
-        // blocks as many per CU as their LDS allows (at most four) in a multiple
-        // of the group count, apply four per CU
-        const int occ_node = std::max(1, std::min(2, (160 * 1024) / std::max(1, b->lds_node)));
-        const int occ_gather = std::max(1, std::min(4, (160 * 1024) / std::max(1, b->lds_gather)));
-        sg.b_node = std::max(1, std::min(sg.n_node, occ_node * b->cu));
-        sg.b_gather = std::max(KOR_NGROUPS, std::min((sg.n_gather + KOR_NGROUPS - 1) / KOR_NGROUPS * KOR_NGROUPS,
-                                                     occ_gather * b->cu));
-        sg.b_apply = std::max(1, std::min(sg.n_apply, 4 * b->cu));
+        // grids: one block per work item (the gather's rounded up to a
+        // multiple of the group count, so item i stays on XCD i % G).  The
+        // kernels keep the grid-stride form; a persistent grid of a few blocks
+        // per CU, walking the list in a static order, balanced the uneven item
+        // costs worse than the hardware's block dispatch (8,192 cells of 24^3:
+        // 101-110 k against 133 k evals/s)
+        sg.b_node = std::max(1, sg.n_node);
+        sg.b_gather = std::max(KOR_NGROUPS, (sg.n_gather + KOR_NGROUPS - 1) / KOR_NGROUPS * KOR_NGROUPS);
+        sg.b_apply = std::max(1, sg.n_apply);
         GDT_CHECK(sg.node_items.ensure(sizeof(int4) * ni.size()) == 0 &&
                       sg.gather_items.ensure(sizeof(int4) * gi.size()) == 0 &&
                       sg.apply_items.ensure(sizeof(int4) * ai.size()) == 0,

@@ -167,8 +167,11 @@ int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells
 int launch_cmp_to_box(gdt_context *h, const double *f_cmp, double *f_box, int cells, int layout);
 int launch_box_to_cmp(gdt_context *h, const double *f_box, double *f_cmp, int cells, int layout);
 // ---- matrix cores (mfma_gemm.hip, matrix_mfma.hip) ---------------------------
+// C (+)= A B^T on the matrix cores; with a workspace of ws_floats FP32
+// values, skinny shapes split K over gridDim.z (gemm_k_splits partials).
 int launch_gemm_nt(hipStream_t s, int mode, int M, int N, int K, const void *Ah, const void *Al, const void *Bh,
-                   const void *Bl, float *C, int ldc, int accumulate);
+                   const void *Bl, float *C, int ldc, int accumulate, float *ws = nullptr, size_t ws_floats = 0);
+int gemm_k_splits(int mode, int M, int N, int K);
 int launch_split_f64(hipStream_t s, const double *x, long n, uint16_t *hi, uint16_t *lo, float *f32);
 int launch_mx_operands(hipStream_t s, const double *b, const double *a, uint16_t *Bh, uint16_t *Bl, float *Bf,
                        uint16_t *Ah, uint16_t *Al, float *Af);

@@ -31,7 +31,8 @@ __device__ __forceinline__ double kmt_log(double x, const double *T)
 {
     int e;
     const double m2 = 2.0 * frexp(x, &e);        // [1, 2), exact
-    const int j = (int)((m2 - 1.0) * 128.0);     // exact product, truncated
+    int j = (int)((m2 - 1.0) * 128.0);           // exact product, truncated: 0..127 for
+    j = j < 0 ? 0 : (j > 127 ? 127 : j);         // finite x > 0 (clamped: table reads stay in bounds)
     const double t = fma(m2, T[KMT_C + j], -1.0);
     const double q = fma(t, fma(t, fma(t, fma(t, fma(t, 1.0 / 7.0, -1.0 / 6.0), 0.2), -0.25), 1.0 / 3.0), -0.5);
     const double lp = fma(t * t, q, t);          // log1p(t)

@@ -538,7 +538,7 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
     }
     __syncthreads();
     if (Xout && b == 0 && threadIdx.x < 5) Xout[threadIdx.x] = Xs[threadIdx.x];
-    if (v < npts) {  // no early return: persistent callers run further items after this one
+    if (v < npts) {  // no early return: work-list callers may run further items after this one
         double q0 = 0.0, L = 0.0;
 #pragma unroll
         for (int g = 0; g < G; g++) {
@@ -574,10 +574,11 @@ __global__ __launch_bounds__(APPLY_THREADS) void k_kor_apply(int npts, int npart
 }
 
 // ---------------------------------------------------------------------------
-// uneven batches: persistent work-list kernels
+// uneven batches: work-list kernels
 // ---------------------------------------------------------------------------
 // Each kernel walks a static list of (class, cell, block, slot) items; block p
-// takes items p, p + gridDim.x, ...  Every item runs exactly the body one
+// takes items p, p + gridDim.x, ... (the launches give every item its own
+// block; the loop keeps any smaller grid correct).  Every item runs exactly the body one
 // block of the single-cell kernel runs, so per-cell results are bitwise those
 // of a non-batched run.  Gather lists keep item i in group i % G, and the
 // grid is a multiple of G, so each group stays on one XCD.  No atomics and no

@@ -7,10 +7,21 @@ int gdt_gemm_nt(int mode, int M, int N, int K, const void *A_hi, const void *A_l
                 const void *B_lo, float *C, int ldc, int accumulate)
 {
     GDT_CHECK(A_hi && B_hi && C && ldc >= N, GDT_E_ARG, "gdt_gemm_nt: bad arguments");
-    int rc = launch_gemm_nt(nullptr, mode, M, N, K, A_hi, A_lo, B_hi, B_lo, C, ldc, accumulate);
-    if (rc) return rc;
-    GDT_HIP(hipDeviceSynchronize());
-    return GDT_OK;
+    // skinny shapes split K over a temporary workspace (this entry is
+    // synchronous, so it is freed on return)
+    DevBuf ws;
+    size_t wf = 0;
+    if (M > 0 && N > 0 && M % 128 == 0 && N % 128 == 0 && K > 0) {
+        const int ks = gemm_k_splits(mode, M, N, K);
+        if (ks > 1 && ws.ensure(sizeof(float) * (size_t)ks * M * N) == 0) wf = (size_t)ks * M * N;
+    }
+    int rc = launch_gemm_nt(nullptr, mode, M, N, K, A_hi, A_lo, B_hi, B_lo, C, ldc, accumulate, ws.as<float>(), wf);
+    if (rc == GDT_OK && hipDeviceSynchronize() != hipSuccess) {
+        gdt_set_error(GDT_E_HIP, "gdt_gemm_nt: %s", hipGetErrorString(hipGetLastError()));
+        rc = GDT_E_HIP;
+    }
+    ws.release();
+    return rc;
 }
 
 int gdt_split_f64(const double *x, long n, uint16_t *hi, uint16_t *lo, float *f32)

@@ -9,7 +9,12 @@
 // Block tile 128 x 128, 256 threads = 4 waves, each wave 64 x 64 = 2 x 2
 // MFMA 32 x 32 tiles; K staged through LDS in slices of 32 (bf16) or 16
 // (fp32), rows padded against bank conflicts.  Shapes must be multiples of
-// the tiles (callers pad).
+// the tiles (callers pad).  Split K: when the tile grid is small and K long
+// (skinny products such as batch x nodes x hidden) and the caller passes a
+// workspace, gridDim.z blocks each take a K range and write an FP32 partial
+// tile; k_gemm_ksum adds the partials in z order (reproducible).
+#include <algorithm>
+
 #include "gdt_internal.h"
 #include "kernel_common.h"
 
@@ -41,9 +46,15 @@ __global__ __launch_bounds__(GM_T) void k_gemm_bf16(int M, int N, int K, const u
                                                      const uint16_t *__restrict__ Al,
                                                      const uint16_t *__restrict__ Bh,
                                                      const uint16_t *__restrict__ Bl, float *__restrict__ C, int ldc,
-                                                     int accumulate)
+                                                     int accumulate, int kc, float *__restrict__ part)
 {
     constexpr int NP = SPLIT ? 2 : 1;
+    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+    if (gridDim.z > 1) {  // partial tile of this K range
+        C = part + (size_t)blockIdx.z * M * N;
+        ldc = N;
+        accumulate = 0;
+    }
     __shared__ uint16_t sA[NP][GM_B * GM_PADB];
     __shared__ uint16_t sB[NP][GM_B * GM_PADB];
     const int m0 = blockIdx.y * GM_B, n0 = blockIdx.x * GM_B;
@@ -56,7 +67,7 @@ __global__ __launch_bounds__(GM_T) void k_gemm_bf16(int M, int N, int K, const u
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += GM_KB) {
+    for (int k0 = kb; k0 < ke; k0 += GM_KB) {
         // global -> LDS: 128 rows x 32 k = 512 chunks of 8 bf16 per operand
         const int q0 = threadIdx.x, q1 = threadIdx.x + GM_T;
         const int r0 = q0 / 4, c0 = (q0 % 4) * 8, r1 = q1 / 4, c1 = (q1 % 4) * 8;
@@ -113,8 +124,14 @@ __global__ __launch_bounds__(GM_T) void k_gemm_bf16(int M, int N, int K, const u
 
 __global__ __launch_bounds__(GM_T) void k_gemm_f32(int M, int N, int K, const float *__restrict__ A,
                                                     const float *__restrict__ B, float *__restrict__ C, int ldc,
-                                                    int accumulate)
+                                                    int accumulate, int kc, float *__restrict__ part)
 {
+    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+    if (gridDim.z > 1) {  // partial tile of this K range
+        C = part + (size_t)blockIdx.z * M * N;
+        ldc = N;
+        accumulate = 0;
+    }
     __shared__ float sA[GM_B * GM_PADF];
     __shared__ float sB[GM_B * GM_PADF];
     const int m0 = blockIdx.y * GM_B, n0 = blockIdx.x * GM_B;
@@ -127,7 +144,7 @@ __global__ __launch_bounds__(GM_T) void k_gemm_f32(int M, int N, int K, const fl
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += GM_KF) {
+    for (int k0 = kb; k0 < ke; k0 += GM_KF) {
         // 128 rows x 16 k = 512 chunks of 4 floats per operand
         float4 ra[2], rb[2];
 #pragma unroll
@@ -204,29 +221,64 @@ int launch_split_f64(hipStream_t s, const double *x, long n, uint16_t *hi, uint1
     return GDT_OK;
 }
 
+// C (+)= sum_z part[z] in z order, elementwise.
+__global__ void k_gemm_ksum(int M, int N, int KS, const float *__restrict__ part, float *__restrict__ C, int ldc,
+                            int accumulate)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long MN = (long)M * N;
+    if (t >= MN) return;
+    float q = 0.0f;
+    for (int z = 0; z < KS; z++) q += part[z * MN + t];
+    float *p = C + (t / N) * ldc + t % N;
+    *p = accumulate ? *p + q : q;
+}
+
+int gemm_k_splits(int mode, int M, int N, int K)
+{
+    const int slice = mode == GDT_GEMM_F32 ? GM_KF : GM_KB;
+    const int blocks = (M / GM_B) * (N / GM_B), slices = K / slice;
+    if (blocks >= 256 || slices < 16) return 1;
+    // at least 8 slices per block, about 512 blocks in all
+    const int ks = std::min(slices / 8, (512 + blocks - 1) / blocks);
+    const int per = (slices + ks - 1) / ks;
+    return (slices + per - 1) / per;  // every split non-empty
+}
+
 int launch_gemm_nt(hipStream_t s, int mode, int M, int N, int K, const void *Ah, const void *Al, const void *Bh,
-                   const void *Bl, float *C, int ldc, int accumulate)
+                   const void *Bl, float *C, int ldc, int accumulate, float *ws, size_t ws_floats)
 {
     GDT_CHECK(M > 0 && N > 0 && K > 0 && M % GM_B == 0 && N % GM_B == 0, GDT_E_ARG,
               "gemm: M, N must be positive multiples of %d (M %d, N %d)", GM_B, M, N);
-    const dim3 grid(N / GM_B, M / GM_B);
+    const int slice = mode == GDT_GEMM_F32 ? GM_KF : GM_KB;
+    GDT_CHECK(K % slice == 0, GDT_E_ARG, "gemm: K %d not a multiple of %d", K, slice);
+    int ks = ws ? gemm_k_splits(mode, M, N, K) : 1;
+    if ((size_t)ks * M * N > ws_floats) ks = 1;
+    const int kc = ((K / slice + ks - 1) / ks) * slice;
+    const dim3 grid(N / GM_B, M / GM_B, ks);
     if (mode == GDT_GEMM_F32) {
-        GDT_CHECK(K % GM_KF == 0, GDT_E_ARG, "gemm f32: K %d not a multiple of %d", K, GM_KF);
         hipLaunchKernelGGL(k_gemm_f32, grid, dim3(GM_T), 0, s, M, N, K, (const float *)Ah, (const float *)Bh, C, ldc,
-                           accumulate);
+                           accumulate, kc, ws);
     } else if (mode == GDT_GEMM_BF16 || mode == GDT_GEMM_BF16_SPLIT) {
-        GDT_CHECK(K % GM_KB == 0, GDT_E_ARG, "gemm bf16: K %d not a multiple of %d", K, GM_KB);
         if (mode == GDT_GEMM_BF16_SPLIT) {
             GDT_CHECK(Al && Bl, GDT_E_ARG, "gemm bf16 split: low parts missing");
             hipLaunchKernelGGL(k_gemm_bf16<true>, grid, dim3(GM_T), 0, s, M, N, K, (const uint16_t *)Ah,
-                               (const uint16_t *)Al, (const uint16_t *)Bh, (const uint16_t *)Bl, C, ldc, accumulate);
+                               (const uint16_t *)Al, (const uint16_t *)Bh, (const uint16_t *)Bl, C, ldc, accumulate,
+                               kc, ws);
         } else {
             hipLaunchKernelGGL(k_gemm_bf16<false>, grid, dim3(GM_T), 0, s, M, N, K, (const uint16_t *)Ah,
-                               (const uint16_t *)Ah, (const uint16_t *)Bh, (const uint16_t *)Bh, C, ldc, accumulate);
+                               (const uint16_t *)Ah, (const uint16_t *)Bh, (const uint16_t *)Bh, C, ldc, accumulate,
+                               kc, ws);
         }
     } else {
         GDT_CHECK(false, GDT_E_ARG, "gemm: unknown mode %d", mode);
     }
     GDT_HIP(hipGetLastError());
+    if (ks > 1) {
+        const long mn = (long)M * N;
+        hipLaunchKernelGGL(k_gemm_ksum, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, M, N, ks,
+                           (const float *)ws, C, ldc, accumulate);
+        GDT_HIP(hipGetLastError());
+    }
     return GDT_OK;
 }

@@ -182,15 +182,26 @@ __global__ void k_sg_dbilinear(int count, const float *__restrict__ dh, const fl
     dwf[t] = b;
 }
 
-// column sums over B rows of a bf16 or fp32 [B][C] array (bias gradients)
-__global__ void k_sg_colsum(int B, int C, const uint16_t *__restrict__ xh, const float *__restrict__ xf,
-                            float *__restrict__ out)
+// column sums over B rows of a bf16 or fp32 [B][C] array (bias gradients):
+// a block of 256 threads takes 8 columns; thread (k, c) adds rows k, k + 32,
+// ... of column c, and the 32 row-slice sums are added in a fixed tree order
+// (reproducible).  Grid: ceil(C / 8) blocks.
+__global__ __launch_bounds__(256) void k_sg_colsum(int B, int C, const uint16_t *__restrict__ xh,
+                                                   const float *__restrict__ xf, float *__restrict__ out)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= C) return;
+    __shared__ float part[32][9];
+    const int cl = threadIdx.x & 7, k = threadIdx.x >> 3, j = blockIdx.x * 8 + cl;
     float s = 0.0f;
-    for (int r = 0; r < B; r++) s += xh ? __uint_as_float((uint32_t)xh[(long)r * C + j] << 16) : xf[(long)r * C + j];
-    out[j] = s;
+    if (j < C)
+        for (int r = k; r < B; r += 32)
+            s += xh ? __uint_as_float((uint32_t)xh[(long)r * C + j] << 16) : xf[(long)r * C + j];
+    part[k][cl] = s;
+    __syncthreads();
+    for (int h = 16; h > 0; h >>= 1) {
+        if (k < h) part[k][cl] += part[k + h][cl];
+        __syncthreads();
+    }
+    if (k == 0 && j < C) out[j] = part[0][cl];
 }
 
 // Adam step on FP32 master weights; refreshes the bf16 copy
@@ -289,13 +300,25 @@ struct gdt_surrogate {
         lossp, idx;
     // inference scratch
     DevBuf dens, Qnn, res, flg, sub_f, sub_Q, sub_nu, sub_g;
+    DevBuf gws;  // split-K partials of the skinny GEMMs
 };
+
+#define SG_FALLBACK_CHUNK 2048  // cells per plain-solver call of the fallback
 
 static int sg_blocks(long n) { return (int)std::max(1L, (n + 255) / 256); }
 
+// C = A Bm^T (bf16 operands, FP32 C [M][N]); skinny products split K into
+// the surrogate's GEMM workspace.
 static int sg_gemm(gdt_surrogate *s, int M, int N, int K, const DevBuf &A, const DevBuf &Bm, DevBuf &C)
 {
-    return launch_gemm_nt(s->h->stream, GDT_GEMM_BF16, M, N, K, A.p, nullptr, Bm.p, nullptr, C.as<float>(), N, 0);
+    const size_t need = (size_t)gemm_k_splits(GDT_GEMM_BF16, M, N, K) * M * N;
+    float *ws = nullptr;
+    if (need > (size_t)M * N) {
+        GDT_CHECK(s->gws.ensure(sizeof(float) * need) == 0, GDT_E_NOMEM, "surrogate: GEMM workspace allocation failed");
+        ws = s->gws.as<float>();
+    }
+    return launch_gemm_nt(s->h->stream, GDT_GEMM_BF16, M, N, K, A.p, nullptr, Bm.p, nullptr, C.as<float>(), N, 0, ws,
+                          ws ? need : 0);
 }
 
 static int sg_transpose(gdt_surrogate *s, int R, int C, const DevBuf &src, DevBuf &dst)
@@ -427,7 +450,7 @@ int gdt_surrogate_destroy(gdt_surrogate_handle s)
                      &s->W3th, &s->mux, &s->sdx, &s->muy, &s->sdy, &s->X, &s->Xt, &s->T, &s->P1, &s->P2, &s->Hh,
                      &s->Hth, &s->Y, &s->dYh, &s->dYth, &s->dH, &s->dUh, &s->dUf, &s->dUth, &s->dWh, &s->dWf,
                      &s->dWth, &s->gW1, &s->gW2, &s->gW3, &s->gb1, &s->gb2, &s->gb3, &s->lossp, &s->idx, &s->dens,
-                     &s->Qnn, &s->res, &s->flg, &s->sub_f, &s->sub_Q, &s->sub_nu, &s->sub_g};
+                     &s->Qnn, &s->res, &s->flg, &s->sub_f, &s->sub_Q, &s->sub_nu, &s->sub_g, &s->gws};
     for (DevBuf *d : all) d->release();
     delete s;
     return GDT_OK;
@@ -500,7 +523,7 @@ int gdt_surrogate_train(gdt_surrogate_handle s, const double *f, const double *Q
             if ((rc = sg_transpose(s, B, H, s->Hh, s->Hth))) return rc;
             // output layer: gW3 = dY^T h, gb3 = sum dY, dh = dY W3
             if ((rc = sg_gemm(s, np, H, B, s->dYth, s->Hth, s->gW3))) return rc;
-            hipLaunchKernelGGL(k_sg_colsum, dim3((np + 255) / 256), dim3(256), 0, st, B, np,
+            hipLaunchKernelGGL(k_sg_colsum, dim3((np + 7) / 8), dim3(256), 0, st, B, np,
                                (const uint16_t *)s->dYh.p, (const float *)nullptr, s->gb3.as<float>());
             if ((rc = sg_gemm(s, B, H, np, s->dYh, s->W3th, s->dH))) return rc;
             // bilinear layer: du = dh w, dw = dh u
@@ -511,9 +534,9 @@ int gdt_surrogate_train(gdt_surrogate_handle s, const double *f, const double *Q
             if ((rc = sg_transpose(s, B, H, s->dWh, s->dWth))) return rc;
             if ((rc = sg_gemm(s, H, np, B, s->dUth, s->Xt, s->gW1))) return rc;
             if ((rc = sg_gemm(s, H, np, B, s->dWth, s->Xt, s->gW2))) return rc;
-            hipLaunchKernelGGL(k_sg_colsum, dim3((H + 255) / 256), dim3(256), 0, st, B, H, (const uint16_t *)nullptr,
+            hipLaunchKernelGGL(k_sg_colsum, dim3((H + 7) / 8), dim3(256), 0, st, B, H, (const uint16_t *)nullptr,
                                (const float *)s->dUf.p, s->gb1.as<float>());
-            hipLaunchKernelGGL(k_sg_colsum, dim3((H + 255) / 256), dim3(256), 0, st, B, H, (const uint16_t *)nullptr,
+            hipLaunchKernelGGL(k_sg_colsum, dim3((H + 7) / 8), dim3(256), 0, st, B, H, (const uint16_t *)nullptr,
                                (const float *)s->dWf.p, s->gb2.as<float>());
             GDT_HIP(hipGetLastError());
             // Adam
@@ -618,20 +641,26 @@ int gdt_surrogate_infer(gdt_surrogate_handle s, const double *f, double *Q, int 
     if (fallback) GDT_HIP(hipMemcpy(fallback, s->flg.p, 4 * (size_t)cells, hipMemcpyDeviceToDevice));
     if (n_fallback) *n_fallback = k;
     if (k == 0) return GDT_OK;
-    // plain solver on the flagged cells (corrected Korobov evaluation)
+    // plain solver on the flagged cells (corrected Korobov evaluation), in
+    // chunks of SG_FALLBACK_CHUNK cells so its scratch stays bounded
     GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_surrogate_infer: %d cells need the plain solver, no stencil built", k);
+    const int ch = std::min(k, SG_FALLBACK_CHUNK);
     DevBuf didx;
     GDT_CHECK(didx.ensure(4 * (size_t)k) == 0 && s->sub_f.ensure(8 * (size_t)k * n) == 0 &&
-                  s->sub_Q.ensure(8 * (size_t)k * n) == 0 && s->sub_nu.ensure(8 * (size_t)k * n) == 0 &&
-                  s->sub_g.ensure(8 * (size_t)k * n) == 0,
+                  s->sub_Q.ensure(8 * (size_t)k * n) == 0 && s->sub_nu.ensure(8 * (size_t)ch * n) == 0 &&
+                  s->sub_g.ensure(8 * (size_t)ch * n) == 0,
               GDT_E_NOMEM, "gdt_surrogate_infer: fallback allocation failed");
     GDT_HIP(hipMemcpyAsync(didx.p, idx.data(), 4 * (size_t)k, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_sg_rows, dim3(sg_blocks((long)k * n)), dim3(256), 0, st, k, n, (const int *)didx.p, f,
                        s->sub_f.as<double>(), 0);
     GDT_HIP(hipGetLastError());
-    if ((rc = gdt_evaluate(h, GDT_METHOD_KOROBOV, s->sub_f.as<double>(), s->sub_nu.as<double>(),
-                           s->sub_g.as<double>(), s->sub_Q.as<double>(), k, GDT_LAYOUT_CELL_MAJOR, GDT_FLAG_CORRECT)))
-        return rc;
+    for (int c0 = 0; c0 < k; c0 += ch) {
+        const int kc = std::min(ch, k - c0);
+        if ((rc = gdt_evaluate(h, GDT_METHOD_KOROBOV, s->sub_f.as<double>() + (size_t)c0 * n, s->sub_nu.as<double>(),
+                               s->sub_g.as<double>(), s->sub_Q.as<double>() + (size_t)c0 * n, kc,
+                               GDT_LAYOUT_CELL_MAJOR, GDT_FLAG_CORRECT)))
+            return rc;
+    }
     hipLaunchKernelGGL(k_sg_rows, dim3(sg_blocks((long)k * n)), dim3(256), 0, st, k, n, (const int *)didx.p,
                        (const double *)s->sub_Q.p, Q, 1);
     GDT_HIP(hipGetLastError());

@@ -222,8 +222,8 @@ int gdt_surrogate_info(gdt_surrogate_handle s, int *n, int *np, int *hidden, lon
  * Classes are handles with a Korobov stencil built; f[s] is device FP64
  * [cells[s]][n_pnt of class s] (cell-major), updated in place by steps.
  * Each run step evaluates every cell (corrected if GDT_FLAG_CORRECT) and,
- * for dt != 0, applies f += dt Q: per segment of cells, three persistent
- * work-list kernels; with GDT_FLAG_GRAPH the whole step is one hipGraph.
+ * for dt != 0, applies f += dt Q: per segment of cells, three work-list
+ * kernels; with GDT_FLAG_GRAPH the whole step is one hipGraph.
  * Every cell's result is bitwise that of gdt_run_steps on the cell alone.
  * The batch keeps the f / Q pointers and the class handles: they must
  * outlive it. */

@@ -20,11 +20,14 @@ def say(msg):
     print(msg, flush=True)
 
 
+@pytest.mark.parametrize("shape", [(256, 384, 512), (256, 128, 4096)])
 @pytest.mark.parametrize("mode,tol", [(G.GEMM_BF16, 1e-2), (G.GEMM_BF16_SPLIT, 2e-5), (G.GEMM_F32, 2e-6)])
-def test_gemm_nt_modes(mode, tol):
-    say("gemm mode %d" % mode)
+def test_gemm_nt_modes(mode, tol, shape):
+    """Both a square-ish product and a skinny one (2 tiles, K 4096: split K
+    over partial tiles added in a fixed order)."""
+    say("gemm mode %d shape %s" % (mode, shape))
     rng = np.random.default_rng(3)
-    M, N, K = 256, 384, 512
+    M, N, K = shape
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((N, K))
     C = G.gemm_nt(mode, G.DeviceArray.from_host(A), G.DeviceArray.from_host(B)).get()
