@@ -67,27 +67,46 @@ __global__ void k_sg_scale(long rows, int n, int np, const double *__restrict__ 
     out[t] = r;
 }
 
-// column mean / standard deviation over rows (two passes, FP64 sums)
-__global__ void k_sg_stats(long rows, int n, int np, const float *__restrict__ x, float *__restrict__ mu,
-                           float *__restrict__ sd)
+// column mean / standard deviation over rows (two passes, FP64 sums): a block
+// of 256 threads takes 8 columns; thread (k, c) sums rows k, k + 32, ... of
+// column c and the 32 row-slice sums are added in a fixed tree order
+// (reproducible).  Columns n .. np-1 (padding) get mean 0, deviation 1.
+// Grid: ceil(np / 8) blocks.
+__global__ __launch_bounds__(256) void k_sg_stats(long rows, int n, int np, const float *__restrict__ x,
+                                                  float *__restrict__ mu, float *__restrict__ sd)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= np) return;
-    if (j >= n) {
-        mu[j] = 0.0f;
-        sd[j] = 1.0f;
-        return;
-    }
+    __shared__ double part[32][9];
+    __shared__ double mean[8];
+    const int cl = threadIdx.x & 7, k = threadIdx.x >> 3, j = blockIdx.x * 8 + cl;
+    const bool live = j < n;
     double s = 0.0;
-    for (long r = 0; r < rows; r++) s += x[r * np + j];
-    const double m = s / (double)rows;
-    double q = 0.0;
-    for (long r = 0; r < rows; r++) {
-        const double d = x[r * np + j] - m;
-        q += d * d;
+    if (live)
+        for (long r = k; r < rows; r += 32) s += x[r * np + j];
+    part[k][cl] = s;
+    __syncthreads();
+    for (int h = 16; h > 0; h >>= 1) {
+        if (k < h) part[k][cl] += part[k + h][cl];
+        __syncthreads();
     }
-    mu[j] = (float)m;
-    sd[j] = (float)std::max(sqrt(q / (double)rows), 1e-12);
+    if (k == 0) mean[cl] = part[0][cl] / (double)rows;
+    __syncthreads();
+    const double m = mean[cl];
+    double q = 0.0;
+    if (live)
+        for (long r = k; r < rows; r += 32) {
+            const double d = x[r * np + j] - m;
+            q += d * d;
+        }
+    part[k][cl] = q;
+    __syncthreads();
+    for (int h = 16; h > 0; h >>= 1) {
+        if (k < h) part[k][cl] += part[k + h][cl];
+        __syncthreads();
+    }
+    if (k == 0 && j < np) {
+        mu[j] = live ? (float)m : 0.0f;
+        sd[j] = live ? (float)std::max(sqrt(part[0][cl] / (double)rows), 1e-12) : 1.0f;
+    }
 }
 
 // standardise in place: x = (x - mu) / sd
@@ -477,9 +496,9 @@ int gdt_surrogate_train(gdt_surrogate_handle s, const double *f, const double *Q
                        (const double *)dens.p, 1, (const float *)nullptr, (const float *)nullptr, Xall.as<float>());
     hipLaunchKernelGGL(k_sg_scale, dim3(sg_blocks(ns * np)), dim3(256), 0, st, ns, n, np, Q,
                        (const double *)dens.p, 2, (const float *)nullptr, (const float *)nullptr, Yall.as<float>());
-    hipLaunchKernelGGL(k_sg_stats, dim3((np + 255) / 256), dim3(256), 0, st, ns, n, np, Xall.as<float>(),
+    hipLaunchKernelGGL(k_sg_stats, dim3((np + 7) / 8), dim3(256), 0, st, ns, n, np, Xall.as<float>(),
                        s->mux.as<float>(), s->sdx.as<float>());
-    hipLaunchKernelGGL(k_sg_stats, dim3((np + 255) / 256), dim3(256), 0, st, ns, n, np, Yall.as<float>(),
+    hipLaunchKernelGGL(k_sg_stats, dim3((np + 7) / 8), dim3(256), 0, st, ns, n, np, Yall.as<float>(),
                        s->muy.as<float>(), s->sdy.as<float>());
     // one scale for all nodes (per-node centring only): the training loss is
     // then the plain L2 error of the centred Q / n^2, the metric the surrogate
