@@ -11,7 +11,7 @@
  * intended behaviour instead of the literal one, the comment says so.
  *
  * Pins (what checks this file independently of the HIP code):
- *   grid / cut ........ closed-form node counts (tests/test_oracle_grid.py)
+ *   grid / cut ........ closed-form node counts (tests/test_oracle.py)
  *   orc_init_f ........ golden F column of true_results.out
  *   orc_loss .......... golden DC column (cells != 0) and the closed-form
  *                       hard-sphere collision frequency of a Maxwellian
@@ -23,7 +23,10 @@
  *   orc_correct ....... residual of the 5x5 system, conserved moments
  *   Korobov ........... lattice permutation identity, exact per-node mass /
  *                       momentum / energy balance, Q(Maxwellian) = 0, loss ->
- *                       closed-form collision frequency, gain -> orc_gain
+ *                       closed-form collision frequency, gain -> orc_gain.
+ *                       The method has no reference counterpart, so these
+ *                       invariants are its only pins: bitwise agreement with
+ *                       an independent implementation is parity unpinned.
  *
  * Build: oracle/Makefile (gcc -O3 -ffp-contract=off -fopenmp).  Contraction is
  * off so that every floating-point step of the integer-selecting code (grid
