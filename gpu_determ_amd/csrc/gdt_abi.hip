@@ -285,9 +285,8 @@ int gdt_create(gdt_handle *out, const gdt_grid_desc *gd, int device)
         gdt_set_error(GDT_E_NOMEM, "gdt_create: device allocation failed");
         rc = GDT_E_NOMEM;
     }
-    if (rc == GDT_OK && (hipStreamCreate(&h->stream) != hipSuccess ||
-                         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)) {
-        gdt_set_error(GDT_E_HIP, "gdt_create: stream/event creation failed");
+    if (rc == GDT_OK && hipStreamCreate(&h->stream) != hipSuccess) {
+        gdt_set_error(GDT_E_HIP, "gdt_create: stream creation failed");
         rc = GDT_E_HIP;
     }
     if (rc == GDT_OK &&
@@ -320,8 +319,6 @@ int gdt_destroy(gdt_handle h)
                       &h->s_out1, &h->s_out2, &h->s_vals, &h->s_loss, &h->s_Q,   &h->s_nu,  &h->s_tmp, &h->s_dt,
                       &h->s_mpart, &h->s_part};
     for (DevBuf *b : bufs) b->release();
-    if (h->ev0) (void)hipEventDestroy(h->ev0);
-    if (h->ev1) (void)hipEventDestroy(h->ev1);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

@@ -99,7 +99,6 @@ struct gdt_context {
     // timing
     bool timing = false;
     std::map<std::string, KernelStat> stats;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // timed launches: events recorded before and after each, back to back on
     // the stream (no wait between launches), resolved when the stats are read
     struct PendingTime {
