@@ -61,30 +61,36 @@ __device__ __forceinline__ uint16_t mx_bf16(float x)
 }
 __device__ __forceinline__ float mx_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Column c of tile t of class p: ix = t / 2, iy = 8 (t % 2) + c / 16,
-// iz = c % 16, s_a = 2 i_a + p_a; -1 when off the class (i_a >= 16 - p_a),
-// else (sx, sy, sz) packed by bytes.
-__device__ __forceinline__ int mx_col(int p, int t, int c)
+// T[n1, s] by parts: for a thread's fixed n1 and tile column iz (sx fixed by
+// the tile), the x and z halves of A = (s + n)/2 and B = (s - n)/2 and their
+// drop tests are computed once; mx_Ty adds the y half of row iy.
+struct MxTxz {
+    int xa, xb, za, zb, ny;
+    bool ok;
+};
+
+__device__ __forceinline__ MxTxz mx_txz(int p, int sx, int iz, int n1)
 {
-    const int px = (p >> 2) & 1, py = (p >> 1) & 1, pz = p & 1;
-    const int ix = t >> 1, iy = 8 * (t & 1) + (c >> 4), iz = c & 15;
-    if (ix >= MX_N - px || iy >= MX_N - py || iz >= MX_N - pz) return -1;
-    return (2 * ix + px) | ((2 * iy + py) << 8) | ((2 * iz + pz) << 16);
+    const int px = (p >> 2) & 1, pz = p & 1;
+    const int nx = 2 * (n1 >> 6) + px, nz = 2 * (n1 & 7) + pz, sz = 2 * iz + pz;
+    const int ax = (sx + nx) >> 1, mx = sx - nx, az = (sz + nz) >> 1, mz = sz - nz;
+    MxTxz t;
+    t.ok = iz < MX_N - pz && ax <= MX_N - 1 && mx >= 0 && az <= MX_N - 1 && mz >= 0;
+    t.xa = ax * MX_N * MX_N;
+    t.xb = (mx >> 1) * MX_N * MX_N;
+    t.za = az;
+    t.zb = mz >> 1;
+    t.ny = 2 * ((n1 >> 3) & 7) + ((p >> 1) & 1);
+    return t;
 }
 
-// T[n1, s] from the LDS copy of f (FP64), with the reference's drop rule.
-__device__ __forceinline__ double mx_T(const double *fs, int p, int sx, int sy, int sz, int n1)
+__device__ __forceinline__ double mx_Ty(const double *fs, const MxTxz &t, int iy, int py)
 {
-    const int nx = 2 * (n1 >> 6) + ((p >> 2) & 1), ny = 2 * ((n1 >> 3) & 7) + ((p >> 1) & 1),
-              nz = 2 * (n1 & 7) + (p & 1);
-    const int ax = (sx + nx) >> 1, ay = (sy + ny) >> 1, az = (sz + nz) >> 1;
-    const int mx = sx - nx, my = sy - ny, mz = sz - nz;
-    if (ax > MX_N - 1 || mx < 0 || ay > MX_N - 1 || my < 0 || az > MX_N - 1 || mz < 0) return 0.0;
-    const int bx = mx >> 1, by = my >> 1, bz = mz >> 1;
-#define MF(x, y, z) fs[((x) * MX_N + (y)) * MX_N + (z)]
-    return MF(ax, ay, az) * MF(bx, by, bz) + MF(ax, by, bz) * MF(bx, ay, az) + MF(ax, ay, bz) * MF(bx, by, az) +
-           MF(ax, by, az) * MF(bx, ay, bz);
-#undef MF
+    const int sy = 2 * iy + py, ay = (sy + t.ny) >> 1, my = sy - t.ny;
+    if (!t.ok || iy >= MX_N - py || ay > MX_N - 1 || my < 0) return 0.0;
+    const int ya = ay * MX_N, yb = (my >> 1) * MX_N;
+    return fs[t.xa + ya + t.za] * fs[t.xb + yb + t.zb] + fs[t.xa + yb + t.zb] * fs[t.xb + ya + t.za] +
+           fs[t.xa + ya + t.zb] * fs[t.xb + yb + t.za] + fs[t.xa + yb + t.za] * fs[t.xb + ya + t.zb];
 }
 
 // MODE: 0 bf16, 1 bf16 split, 2 fp32.  Grid (8 classes, cells); AwT[p][g][n1]
@@ -103,7 +109,6 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
     float *fA = (float *)ops;                            // [512][MX_PF]
     float *fB = fA + MX_G * MX_PF;                       // [128][MX_PF]
     float *Gc = (float *)ops;                            // [512][MX_GP]: 32 columns of G
-    int *scol = (int *)(ops + mx_ops_bytes(MODE));       // the tile's columns (mx_col)
     const int p = blockIdx.x, c = blockIdx.y;
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     const double *fc = f + (size_t)c * MX_NB;
@@ -121,11 +126,9 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
     // [0, 15], i.e. iz in [izlo, izlo + 7] (at most 8 values of step 2)
     const int izlo = (vz - pz + 1) >> 1;
     for (int t = 0; t < NT; t++) {
+        // tile t: sx = 2 (t / 2) + px; column c: iy = 8 (t % 2) + c / 16,
+        // iz = c % 16 (off the class where i_a >= 16 - p_a: T = 0)
         const int sx_t = 2 * (t >> 1) + px;
-        // the tile's column coordinates, once (the K loop's first barrier
-        // publishes them); the previous tile's reads are done first
-        __syncthreads();
-        if (threadIdx.x < MX_S) scol[threadIdx.x] = mx_col(p, t, threadIdx.x);
         f32x16 acc[2][4];
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -154,23 +157,12 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
                 // and B = (s - n) / 2) are fixed; row by row sy grows by 2, so
                 // Ay and By step by one.
                 {
-                    const int nk = threadIdx.x % KB, iz = threadIdx.x / KB, n1 = k0 + nk;
-                    const int nx = 2 * (n1 >> 6) + px, ny = 2 * ((n1 >> 3) & 7) + py, nz = 2 * (n1 & 7) + pz;
-                    const int sz = 2 * iz + pz;
-                    const int ax = (sx_t + nx) >> 1, mx = sx_t - nx, az = (sz + nz) >> 1, mz = sz - nz;
-                    const bool okxz = iz < MX_N - pz && ax <= MX_N - 1 && mx >= 0 && az <= MX_N - 1 && mz >= 0;
-                    const int xa = ax * MX_N * MX_N, xb = (mx >> 1) * MX_N * MX_N, za = az, zb = mz >> 1;
+                    const int nk = threadIdx.x % KB, iz = threadIdx.x / KB;
+                    const MxTxz tx = mx_txz(p, sx_t, iz, k0 + nk);
                     const int iy0 = 8 * (t & 1);
 #pragma unroll 1
                     for (int i = 0; i < 8; i++) {
-                        const int iy = iy0 + i, sy = 2 * iy + py;
-                        const int ay = (sy + ny) >> 1, my = sy - ny;
-                        double tv = 0.0;
-                        if (okxz && iy < MX_N - py && ay <= MX_N - 1 && my >= 0) {
-                            const int ya = ay * MX_N, yb = (my >> 1) * MX_N;
-                            tv = fs[xa + ya + za] * fs[xb + yb + zb] + fs[xa + yb + zb] * fs[xb + ya + za] +
-                                 fs[xa + ya + zb] * fs[xb + yb + za] + fs[xa + yb + za] * fs[xb + ya + zb];
-                        }
+                        const double tv = mx_Ty(fs, tx, iy0 + i, py);
                         const int sc = 16 * i + iz;
                         const uint16_t h = mx_bf16((float)tv);
                         sB[sc * MX_PB + nk] = h;
@@ -188,12 +180,13 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
                     d[2] = v.z;
                     d[3] = v.w;
                 }
-#pragma unroll 2
+                // 128 s x 16 n1, 4 per thread: n1 = k0 + q % 16 and the column
+                // iz = (q / 16) % 16 fixed, rows iy0 + q / 256 + 2 i
+                const MxTxz tx = mx_txz(p, sx_t, (threadIdx.x >> 4) & 15, k0 + (threadIdx.x & 15));
+#pragma unroll 1
                 for (int i = 0; i < 4; i++) {
-                    const int q = threadIdx.x + i * MX_T, sc = q / KB, nk = q % KB;
-                    double tv = 0.0;
-                    const int pk = scol[sc];
-                    if (pk >= 0) tv = mx_T(fs, p, pk & 255, (pk >> 8) & 255, pk >> 16, k0 + nk);
+                    const int sc = (threadIdx.x >> 4) + 32 * i, nk = threadIdx.x & 15;
+                    const double tv = mx_Ty(fs, tx, 8 * (t & 1) + (threadIdx.x >> 8) + 2 * i, py);
                     fB[sc * MX_PF + nk] = (float)tv;
                 }
             }
@@ -341,11 +334,11 @@ int launch_mx_operands(hipStream_t s, const double *b, const double *a, uint16_t
     return GDT_OK;
 }
 
-// f (FP64), the operand region, the tile's column coordinates.
+// f (FP64) and the operand region.
 static size_t mx_lds_bytes(int mode)
 {
     const int m = mode == GDT_GEMM_F32 ? 2 : (mode == GDT_GEMM_BF16_SPLIT ? 1 : 0);
-    return 8 * MX_NB + mx_ops_bytes(m) + sizeof(int) * MX_S;
+    return 8 * MX_NB + mx_ops_bytes(m);
 }
 
 int launch_mx_gain(hipStream_t s, int mode, const uint16_t *Ah, const uint16_t *Al, const float *Af, const double *f,
