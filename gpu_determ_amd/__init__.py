@@ -44,7 +44,7 @@ ABI_SYMBOLS = (
     "gdt_gemm_nt", "gdt_split_f64", "gdt_matrix_mfma",
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
-    "gdt_init_mixture", "gdt_korobov_tiles", "gdt_kmath_pow",
+    "gdt_init_mixture", "gdt_korobov_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
 )
 
 _lib = None
@@ -127,6 +127,7 @@ def lib():
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
         "gdt_korobov_tiles": [vp, ip, ip, ip],
         "gdt_kmath_pow": [vp, vp, vp, vp, C.c_long],
+        "gdt_kmath_tables": [vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
@@ -479,6 +480,13 @@ def gemm_nt(mode, A, B, accumulate=None):
     else:
         check(lib().gdt_gemm_nt(mode, M, N, K, ah.ptr, al.ptr, bh.ptr, bl.ptr, C_.ptr, N, int(accumulate is not None)))
     return C_
+
+
+def kmath_tables():
+    """The node terms' log / exp tables (host, no device needed)."""
+    T = np.zeros(520)
+    check(lib().gdt_kmath_tables(T.ctypes.data))
+    return T
 
 
 def korobov_pick(M, seed, ncand=8):

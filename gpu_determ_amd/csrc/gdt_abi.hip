@@ -1048,6 +1048,13 @@ int gdt_init_mixture(gdt_handle h, const double *params, int comps, long cells, 
     return launch_init_mixture(h, params, comps, cells, f);
 }
 
+int gdt_kmath_tables(double *T)
+{
+    GDT_CHECK(T, GDT_E_ARG, "gdt_kmath_tables: null pointer");
+    kmath_tables(T);
+    return GDT_OK;
+}
+
 int gdt_kmath_pow(gdt_handle h, const double *x, const double *r, double *out, long n)
 {
     GDT_CHECK(h && x && r && out && n >= 0, GDT_E_ARG, "gdt_kmath_pow: bad arguments");

@@ -127,6 +127,10 @@ int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
  * terms compute it (table-driven FP64 log and exp); device pointers, x > 0;
  * synchronous. */
 int gdt_kmath_pow(gdt_handle h, const double *x, const double *r, double *out, long n);
+/* Host copy of those log / exp tables (520 doubles: c_j, -log c_j high and
+ * low for j < 128; 2^(j/64) high and low for j < 64; ln2 high and low, 64 /
+ * ln2, ln2 / 64 high and low; zero padding).  No device needed. */
+int gdt_kmath_tables(double *T);
 /* Host copy of the CSR: ptr[n_pnt+1], ent[entries] = node*4 + role. */
 int gdt_korobov_export_csr(gdt_handle h, int32_t *ptr, uint32_t *ent);
 /* Generator choice alone (host): same rule as gdt_korobov_build. */
