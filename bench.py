@@ -6,10 +6,11 @@ velocity grid cut to the sphere |v| < vmax = 4.8 (17,256 nodes), FP64, plain
 Korobov/projection method with 2^21 lattice nodes, one spatial cell whose f is
 a seeded mixture of 3 Maxwellians (n ~ U(0.2,1), T ~ U(0.3,1.5), |u| <= 1.2),
 explicit steps df/dt = Q(f,f) with dt = 0.02 and the conservation correction
-every step.  One step = one corrected Q(f,f) evaluation + the Euler update,
-replayed as a hipGraph.  Metric: Q(f,f) evaluations per second.
+every step.  One step = one corrected Q(f,f) evaluation + the Euler update:
+three kernels launched onto the stream (--graph: replayed as a hipGraph).
+Metric: Q(f,f) evaluations per second.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 5]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--graph]
 
 Prints ONE JSON line.  The roofline object reports the dominant kernel's
 algorithmic bytes over its hipEvent-timed average duration (a separate
@@ -78,8 +79,9 @@ def batch_fs(xi, cells, rng):
 def uneven_batch(G, cells=10000, steps=3, seed=SEED):
     """Uneven batch: cells on grids drawn from {16,20,24,32}^3 (vmax 4.8,
     sphere cut), one 2^20-node stencil per grid size, every cell corrected
-    each step, persistent work-list launches captured as one hipGraph, launch
-    shape from the autotuner.  Returns the measured figures."""
+    each step, work-list launches captured as one hipGraph (and, beside it,
+    launched straight onto the stream), launch shape from the autotuner.
+    Returns the measured figures."""
     rng = np.random.default_rng(seed + 4)
     sizes = (16, 20, 24, 32)
     pick = rng.integers(0, len(sizes), cells)
@@ -102,9 +104,17 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
     b.run(DT, steps, flags)
     b.sync()
     el = time.perf_counter() - t0
+    # the same steps launched straight onto the stream (no graph), beside it
+    b.run(DT, 1, G.FLAG_CORRECT)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(DT, steps, G.FLAG_CORRECT)
+    b.sync()
+    el_eager = time.perf_counter() - t0
     out = {"cells": cells, "grids": {"%d^3" % n: int(np.sum(pick == i)) for i, n in enumerate(sizes)},
            "korobov_M": 1 << 20, "steps": steps, "evals_per_s": round(cells * steps / el, 1),
-           "ms_per_step": round(1e3 * el / steps, 3), "tuned": {"seg_cells": seg, "nodes_per_thread": q,
+           "ms_per_step": round(1e3 * el / steps, 3), "eager_evals_per_s": round(cells * steps / el_eager, 1),
+           "tuned": {"seg_cells": seg, "nodes_per_thread": q,
                                                                 "best_eval_ms": round(tune_ms, 3)},
            "setup_s": round(setup_s, 2), "tune_s": round(tune_s, 2)}
     b.close()
@@ -299,6 +309,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roofline", action="store_true", help="skip the instrumented kernel-timing pass")
     ap.add_argument("--fused", action="store_true", help="per-entry recomputation path (GDT_FLAG_FUSED)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the steps as a hipGraph (GDT_FLAG_GRAPH) instead of launching them on the stream")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch extension figure")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
@@ -319,7 +331,11 @@ def main():
     f0 = mixture(ctx.xi, rng)
     fd = G.DeviceArray.from_host(f0[None])
 
-    flags = G.FLAG_CORRECT | G.FLAG_GRAPH | (G.FLAG_FUSED if args.fused else 0)
+    # steps are launched straight onto the context stream by default: on this
+    # stack a hipGraph replay of the same three kernels per step costs about
+    # 1.7 us more per kernel (36.5 against 31.5 us per step); --graph measures
+    # the replay
+    flags = G.FLAG_CORRECT | (G.FLAG_GRAPH if args.graph else 0) | (G.FLAG_FUSED if args.fused else 0)
     ctx.run_steps(G.METHOD_KOROBOV, fd, 1, DT, args.warmup, flags)
     ctx.sync()
     f_start = fd.get()[0]
@@ -408,7 +424,7 @@ def main():
         "config": {"workload": "plain FP64 Korobov/projection Q(f,f), full relaxation step", "grid": "32^3",
                    "vmax": VMAX, "nodes": n, "korobov_M": M_NODES, "korobov_a": info["a"],
                    "stencil_nodes": nn, "csr_entries": ne, "cells": 1, "dt": DT, "s0": S0,
-                   "correction": True, "graph": True, "stencil_build_s": round(build_s, 3)},
+                   "correction": True, "graph": bool(args.graph), "stencil_build_s": round(build_s, 3)},
         "checks": {"moment_drift_rel": cons, "entropy_start": h0, "entropy_end": h1,
                    "min_f": float(f_end.min())},
         "roofline": roof,

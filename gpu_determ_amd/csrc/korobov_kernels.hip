@@ -100,6 +100,13 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, 
 // than two per thread) are handled one at a time.
 // Block b of a grid of B blocks per cell: nodes b*KOR_THREADS + tid (+ k B
 // KOR_THREADS) of the cell whose f is fc; values to out (the cell's V).
+typedef unsigned int u32x4_nt __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 load_nt_u4(const uint4 *p)
+{
+    const u32x4_nt v = __builtin_nontemporal_load((const u32x4_nt *)p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const uint4 *__restrict__ nrec,
                                            const uint32_t *__restrict__ nd2, double K, const double *__restrict__ fc,
                                            double2 *__restrict__ out, int b, int B, const double *__restrict__ kmt)
@@ -109,13 +116,15 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
     const long ta = (long)b * KOR_THREADS + threadIdx.x, tb = ta + stride;
     uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
     uint32_t ya = 0u, yb = 0u;
+    // streaming (nontemporal) loads: records are read once per step, and
+    // should not push the gather's entries out of L2
     if (ta < nn) {
-        xa = nrec[ta];
-        ya = nd2[ta];
+        xa = load_nt_u4(&nrec[ta]);
+        ya = __builtin_nontemporal_load(&nd2[ta]);
     }
     if (tb < nn) {
-        xb = nrec[tb];
-        yb = nd2[tb];
+        xb = load_nt_u4(&nrec[tb]);
+        yb = __builtin_nontemporal_load(&nd2[tb]);
     }
     stage_f(fs, fc, npts, kmt);
     __syncthreads();
@@ -484,8 +493,8 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
     double qg[G], lg[G];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        qg[g] = Qp[(size_t)g * npts + vc];
-        lg[g] = Lp[(size_t)g * npts + vc];
+        qg[g] = __builtin_nontemporal_load(&Qp[(size_t)g * npts + vc]);
+        lg[g] = __builtin_nontemporal_load(&Lp[(size_t)g * npts + vc]);
     }
     if (CORR) {
         // block sums of the moments: thread t adds blocks t, t + 256, ...
