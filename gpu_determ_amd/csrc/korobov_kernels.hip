@@ -475,7 +475,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // All global loads of a thread (its node's partials, f and centre, and its
 // share of the moment sums) are issued together at the start, so the block
 // waits for memory once.
-#define APPLY_ROUNDS 4  // block sums per thread and moment with loads in flight (1024 blocks: one round)
+#define APPLY_ROUNDS 2  // pairs of block sums per thread and moment in flight (1024 blocks: one round)
 #define APPLY_THREADS 256
 // Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
 // mp [20][nparts], fc [npts], Lp / Qp [G][npts], outputs [npts], X [5].
@@ -504,18 +504,28 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
         double m[20];
 #pragma unroll
         for (int a = 0; a < 20; a++) m[a] = 0.0;
-        for (int p0 = 0; p0 < nparts; p0 += APPLY_ROUNDS * APPLY_THREADS) {
-            double vv[APPLY_ROUNDS][20];
+        if ((nparts & 1) == 0) {
+            // pairs of partials per 16-byte load (even nparts: every moment
+            // row starts 16-byte aligned and holds whole pairs)
+            for (int p0 = 0; p0 < nparts; p0 += 2 * APPLY_ROUNDS * APPLY_THREADS) {
+                double2 vv[APPLY_ROUNDS][20];
 #pragma unroll
-            for (int k = 0; k < APPLY_ROUNDS; k++) {
-                const int p = p0 + k * APPLY_THREADS + threadIdx.x;
+                for (int k = 0; k < APPLY_ROUNDS; k++) {
+                    const int p = p0 + 2 * (k * APPLY_THREADS + threadIdx.x);
 #pragma unroll
-                for (int a = 0; a < 20; a++) vv[k][a] = p < nparts ? mp[(size_t)a * nparts + p] : 0.0;
+                    for (int a = 0; a < 20; a++)
+                        vv[k][a] = p < nparts ? *(const double2 *)(mp + (size_t)a * nparts + p)
+                                              : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int k = 0; k < APPLY_ROUNDS; k++)
+#pragma unroll
+                    for (int a = 0; a < 20; a++) m[a] += vv[k][a].x + vv[k][a].y;
             }
+        } else {
+            for (int p = threadIdx.x; p < nparts; p += APPLY_THREADS)
 #pragma unroll
-            for (int k = 0; k < APPLY_ROUNDS; k++)
-#pragma unroll
-                for (int a = 0; a < 20; a++) m[a] += vv[k][a];
+                for (int a = 0; a < 20; a++) m[a] += mp[(size_t)a * nparts + p];
         }
 #pragma unroll
         for (int a = 0; a < 20; a++) m[a] += dpp_f64<DPP_QUAD_XOR1>(m[a]);

@@ -240,12 +240,13 @@ def test_korobov_fused_equals_node_gather():
     run-to-run reproducible bit for bit."""
     say("korobov fused vs node+gather")
     vm = 4.8
-    for n, M in (((20, 20, 20), 1 << 18), ((32, 32, 32), 1 << 19)):
+    # 3 cells: the fused grid has 256 / 3 = 85 blocks, an odd count of moment
+    # partials (the apply's one-at-a-time reduction path)
+    for n, M, cells in (((20, 20, 20), 1 << 18, 2), ((32, 32, 32), 1 << 19, 3)):
         g = O.Grid(n, (-vm,) * 3, (vm,) * 3, vm)
         c = ctx_for(g)
         c.korobov_build(M, seed=21, s0=0.25)
         rng = np.random.default_rng(8)
-        cells = 2
         f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
         fd = G.DeviceArray.from_host(f)
         nu1, g1, q1 = c.evaluate(G.METHOD_KOROBOV, fd, cells, 0)
