@@ -220,6 +220,9 @@ int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Q
 #define KOR_TILE_DST 512  // destinations per gather tile: one per thread
 #define KOR_TILE_CAP 4864 // entries per gather pass (76 KB of LDS at most)
 #define KOR_TILES_MAX 128 // gather tiles per group: G * 128 blocks, four per CU, one round
+// LDS copy of f padded to whole 128-double chunks (the node kernels' 16-byte
+// staging writes whole wave chunks); the log / exp tables follow the padding
+__host__ __device__ constexpr int kor_fpad(int npts) { return (npts + 127) & ~127; }
 int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells);
 // G group partials [cell][G][node] of Q and loss (G = 1: plain arrays)
 int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const double *mpart, const double *f,

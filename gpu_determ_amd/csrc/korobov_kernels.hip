@@ -66,27 +66,49 @@ __device__ __forceinline__ void node_terms(const uint4 x, const uint32_t y, cons
     W2P = 2.0 * (w * P);
 }
 
-// Stages f of cell c into LDS fs[0, npts) and the log / exp tables after it,
-// fs[npts, npts + KMT_SIZE).  Every load of the thread is issued before the
-// first store, so the staging costs one memory round trip (npts <=
-// KOR_STAGE * KOR_THREADS = 19,456; the stencil build admits at most 19,200).
+// Stages f of cell c into LDS fs[0, npts) and the log / exp tables at
+// fs[kor_fpad(npts), + KMT_SIZE); the gap is padding that the staging may
+// overwrite.  Returns with every staging write done (the caller's barrier
+// then publishes them).
+//   16-byte path (f 16-byte aligned, npts even): LDS DMA.  Wave w copies the
+//   64-pair chunks w, w + KOR_WAVES, ... with global_load_lds_dwordx4, which
+//   writes lane i's 16 bytes to the chunk's LDS base + 16 i without passing
+//   through registers; lanes past the end re-read the last pair into the
+//   padding (kor_fpad rounds up to whole chunks).  About 9 loads per thread
+//   instead of 17 register round trips, and nothing waits between them.
+//   Otherwise: 8-byte loads, every one of the thread's issued before the
+//   first LDS store (npts <= KOR_STAGE * KOR_THREADS = 19,456; the stencil
+//   build admits at most 19,200).
 #define KOR_STAGE 19
 static_assert(KMT_SIZE <= KOR_THREADS, "one table entry per thread");
+typedef __attribute__((address_space(1))) void gmem_void;
+typedef __attribute__((address_space(3))) void lds_void;
 __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, const double *kmt)
 {
-    double v[KOR_STAGE];
-#pragma unroll
-    for (int j = 0; j < KOR_STAGE; j++) {
-        const int t = j * KOR_THREADS + threadIdx.x;
-        v[j] = fc[t < npts ? t : npts - 1];  // unconditional (clamped) loads: no branches
-    }
     const double tv = kmt[threadIdx.x < KMT_SIZE ? threadIdx.x : 0];
+    if ((npts & 1) == 0 && ((uintptr_t)fc & 15) == 0) {
+        const int np2 = npts >> 1, nch = (np2 + WAVE - 1) / WAVE;
+        const int lane = threadIdx.x % WAVE;
+        for (int c = threadIdx.x / WAVE; c < nch; c += KOR_WAVES) {
+            const int t = c * WAVE + lane;
+            __builtin_amdgcn_global_load_lds((gmem_void *)(fc + 2 * (t < np2 ? t : np2 - 1)),
+                                             (lds_void *)(fs + 2 * WAVE * c), 16, 0, 0);
+        }
+    } else {
+        double v[KOR_STAGE];
 #pragma unroll
-    for (int j = 0; j < KOR_STAGE; j++) {
-        const int t = j * KOR_THREADS + threadIdx.x;
-        if (t < npts) fs[t] = v[j];
+        for (int j = 0; j < KOR_STAGE; j++) {
+            const int t = j * KOR_THREADS + threadIdx.x;
+            v[j] = fc[t < npts ? t : npts - 1];  // unconditional (clamped) loads: no branches
+        }
+#pragma unroll
+        for (int j = 0; j < KOR_STAGE; j++) {
+            const int t = j * KOR_THREADS + threadIdx.x;
+            if (t < npts) fs[t] = v[j];
+        }
     }
-    if (threadIdx.x < KMT_SIZE) fs[npts + threadIdx.x] = tv;
+    if (threadIdx.x < KMT_SIZE) fs[kor_fpad(npts) + threadIdx.x] = tv;
+    __builtin_amdgcn_s_waitcnt(0);  // LDS DMA writes complete under vmcnt
 }
 
 // ---------------------------------------------------------------------------
@@ -94,12 +116,14 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, 
 // ---------------------------------------------------------------------------
 // Per node t of cell c, the values the destination entries read:
 //   V[t] = (D, 2 w P)
-// A thread owns nodes t and t + stride and evaluates both together, so the two
-// dependency chains (LDS reads, log / exp) overlap; their records are loaded
-// before the staging of f, so that latency overlaps too.  Further nodes (more
-// than two per thread) are handled one at a time.
-// Block b of a grid of B blocks per cell: nodes b*KOR_THREADS + tid (+ k B
-// KOR_THREADS) of the cell whose f is fc; values to out (the cell's V).
+// Block b of a grid of B blocks per cell owns the contiguous node range
+// [b per, (b + 1) per), per = ceil(nn / B), so every block has the same share
+// (within one node per thread).  A thread owns nodes t and t + KOR_THREADS of
+// the range and evaluates both together, so the two dependency chains (LDS
+// reads, log / exp) overlap; their records are loaded before the staging of
+// f, so that latency overlaps too.  Further nodes (more than two per thread)
+// are handled one at a time.  f of the cell is fc; values go to out (the
+// cell's V).
 typedef unsigned int u32x4_nt __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint4 load_nt_u4(const uint4 *p)
 {
@@ -112,37 +136,38 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
                                            double2 *__restrict__ out, int b, int B, const double *__restrict__ kmt)
 {
 #pragma clang fp contract(off)
-    const long stride = (long)B * KOR_THREADS;
-    const long ta = (long)b * KOR_THREADS + threadIdx.x, tb = ta + stride;
+    const long per = (nn + B - 1) / B, t0 = (long)b * per;
+    const long tend = t0 + per < nn ? t0 + per : nn;
+    const long ta = t0 + threadIdx.x, tb = ta + KOR_THREADS;
     uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
     uint32_t ya = 0u, yb = 0u;
     // streaming (nontemporal) loads: records are read once per step, and
     // should not push the gather's entries out of L2
-    if (ta < nn) {
+    if (ta < tend) {
         xa = load_nt_u4(&nrec[ta]);
         ya = __builtin_nontemporal_load(&nd2[ta]);
     }
-    if (tb < nn) {
+    if (tb < tend) {
         xb = load_nt_u4(&nrec[tb]);
         yb = __builtin_nontemporal_load(&nd2[tb]);
     }
     stage_f(fs, fc, npts, kmt);
     __syncthreads();
-    const double *ts = fs + npts;
+    const double *ts = fs + kor_fpad(npts);
     double Da, Wa, ra, Db, Wb, rb;
     node_terms(xa, ya, fs, ts, K, Da, Wa, ra);
     node_terms(xb, yb, fs, ts, K, Db, Wb, rb);
     // streaming (nontemporal) stores: the values are read by the next kernel
     // only, from other XCDs, so they need not linger in this XCD's L2
-    if (ta < nn) {
+    if (ta < tend) {
         __builtin_nontemporal_store(Da, &out[ta].x);
         __builtin_nontemporal_store(Wa, &out[ta].y);
     }
-    if (tb < nn) {
+    if (tb < tend) {
         __builtin_nontemporal_store(Db, &out[tb].x);
         __builtin_nontemporal_store(Wb, &out[tb].y);
     }
-    for (long t = tb + stride; t < nn; t += stride) {
+    for (long t = tb + KOR_THREADS; t < tend; t += KOR_THREADS) {
         double D, W2P, r;
         node_terms(nrec[t], nd2[t], fs, ts, K, D, W2P, r);
         __builtin_nontemporal_store(D, &out[t].x);
@@ -404,7 +429,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
     }
     stage_f(fs, f + (size_t)c * npts, npts, kmt);
     __syncthreads();
-    const double *ts = fs + npts;
+    const double *ts = fs + kor_fpad(npts);
     double *Qc = Q + (size_t)c * npts;
     double *Lc = loss + (size_t)c * npts;
     double m[20];
@@ -703,8 +728,9 @@ int launch_kmath_pow(gdt_context *h, const double *x, const double *r, double *o
     return GDT_OK;
 }
 
-// Dynamic LDS of the kernels that stage f: f and the log / exp tables.
-int kor_node_lds(const gdt_context *h) { return (int)(sizeof(double) * (h->n_pnt + KMT_SIZE)); }
+// Dynamic LDS of the kernels that stage f: f (padded to whole 128-double
+// chunks) and the log / exp tables.
+int kor_node_lds(const gdt_context *h) { return (int)(sizeof(double) * (kor_fpad(h->n_pnt) + KMT_SIZE)); }
 
 int kor_blocks(gdt_context *h, int cells)
 {
