@@ -587,6 +587,11 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
         const int G = KOR_NGROUPS;
         std::vector<int32_t> gp((size_t)G * (np + 1), 0);
         auto grp = [&](uint32_t p) { return (int)(((uint64_t)p * G) / (uint64_t)nn); };
+        for (int g = 0; g < G; g++) {  // the node kernel's XCD-aligned ranges follow the same rule
+            const long lo = kor_group_begin(g, nn), hi = kor_group_begin(g + 1, nn);
+            GDT_CHECK(lo >= hi || (grp((uint32_t)lo) == g && grp((uint32_t)(hi - 1)) == g), GDT_E_STATE,
+                      "gdt_korobov_build: group %d range [%ld, %ld) off the group rule", g, lo, hi);
+        }
         for (long p = 0; p < nn; p++) {
             const long t = order[p];
             for (int sl = 0; sl < slots(t); sl++) gp[(size_t)grp(p) * (np + 1) + cidx6[6 * t + sl] + 1]++;

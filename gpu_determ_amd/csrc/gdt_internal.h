@@ -220,6 +220,12 @@ int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Q
 #define KOR_TILE_DST 512  // destinations per gather tile: one per thread
 #define KOR_TILE_CAP 4864 // entries per gather pass (76 KB of LDS at most)
 #define KOR_TILES_MAX 128 // gather tiles per group: G * 128 blocks, four per CU, one round
+// First node position of gather group g (node p is in group p * G / nn, so
+// group g is [kor_group_begin(g), kor_group_begin(g + 1)))
+__host__ __device__ constexpr long kor_group_begin(int g, long nn)
+{
+    return ((long)g * nn + KOR_NGROUPS - 1) / KOR_NGROUPS;
+}
 // LDS copy of f padded to whole 128-double chunks (the node kernels' 16-byte
 // staging writes whole wave chunks); the log / exp tables follow the padding
 __host__ __device__ constexpr int kor_fpad(int npts) { return (npts + 127) & ~127; }

@@ -116,9 +116,15 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, 
 // ---------------------------------------------------------------------------
 // Per node t of cell c, the values the destination entries read:
 //   V[t] = (D, 2 w P)
-// Block b of a grid of B blocks per cell owns the contiguous node range
-// [b per, (b + 1) per), per = ceil(nn / B), so every block has the same share
-// (within one node per thread).  A thread owns nodes t and t + KOR_THREADS of
+// Block b of a grid of B blocks per cell owns a contiguous node range:
+//   B a multiple of KOR_GROUPS (xcd): the range is slice b / G of the S = B / G
+//     equal slices of gather group g = b % G.  Workgroups are dealt to the
+//     XCDs round robin, so the values of group g are written from the XCD
+//     whose L2 the gather of group g then reads them through (one step:
+//     29.0 -> 26.7 us against group-blind ranges);
+//   otherwise: [b per, (b + 1) per), per = ceil(nn / B).
+// Either way every block has the same share (within one node per thread).  A
+// thread owns nodes t and t + KOR_THREADS of
 // the range and evaluates both together, so the two dependency chains (LDS
 // reads, log / exp) overlap; their records are loaded before the staging of
 // f, so that latency overlaps too.  Further nodes (more than two per thread)
@@ -133,11 +139,20 @@ __device__ __forceinline__ uint4 load_nt_u4(const uint4 *p)
 
 __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const uint4 *__restrict__ nrec,
                                            const uint32_t *__restrict__ nd2, double K, const double *__restrict__ fc,
-                                           double2 *__restrict__ out, int b, int B, const double *__restrict__ kmt)
+                                           double2 *__restrict__ out, int b, int B, const double *__restrict__ kmt,
+                                           bool xcd)
 {
 #pragma clang fp contract(off)
-    const long per = (nn + B - 1) / B, t0 = (long)b * per;
-    const long tend = t0 + per < nn ? t0 + per : nn;
+    long lo = 0, hi = nn, k = b, S = B;
+    if (xcd) {
+        const int g = b % KOR_GROUPS;
+        lo = kor_group_begin(g, nn);
+        hi = kor_group_begin(g + 1, nn);
+        k = b / KOR_GROUPS;
+        S = B / KOR_GROUPS;
+    }
+    const long per = (hi - lo + S - 1) / S, t0 = lo + k * per;
+    const long tend = t0 + per < hi ? t0 + per : hi;
     const long ta = t0 + threadIdx.x, tb = ta + KOR_THREADS;
     uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
     uint32_t ya = 0u, yb = 0u;
@@ -182,7 +197,8 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, con
 {
     extern __shared__ double fs[];
     const int c = blockIdx.y;
-    node_block(fs, npts, nn, nrec, nd2, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x, kmt);
+    node_block(fs, npts, nn, nrec, nd2, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x, kmt,
+               gridDim.x % KOR_GROUPS == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -636,7 +652,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_batch_node(const KorClassDev *_
         const int4 it = items[i];
         const KorClassDev C = cls[it.x];
         node_block(fs, C.npts, C.nn, C.nrec, C.nd2, C.K, C.f + (size_t)it.y * C.npts, C.V + (size_t)it.w * C.nn,
-                   it.z, C.node_blocks, C.kmt);
+                   it.z, C.node_blocks, C.kmt, false);
     }
 }
 
@@ -737,7 +753,9 @@ int kor_blocks(gdt_context *h, int cells)
     (void)h;
     const int target = 256;  // one block per CU: one block's LDS per CU
     const int b = (target + cells - 1) / cells;
-    return b < 1 ? 1 : b;
+    // from KOR_GROUPS blocks per cell up: a multiple of KOR_GROUPS, so the node
+    // kernel's ranges follow the gather groups' XCDs
+    return b < KOR_GROUPS ? b : (b + KOR_GROUPS - 1) / KOR_GROUPS * KOR_GROUPS;
 }
 
 // Destination ranges per wave with balanced entry counts, for B blocks.
