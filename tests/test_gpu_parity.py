@@ -191,6 +191,47 @@ def test_korobov_collide_matches_oracle(cells):
     assert np.array_equal(l2.get().reshape(g.n_pnt, cells).T, loss.get())
 
 
+class _DeviceView:
+    """A device pointer into another DeviceArray (which keeps ownership)."""
+
+    def __init__(self, base, offset_doubles):
+        self.base = base
+        self.ptr = base.ptr + 8 * offset_doubles
+
+
+@pytest.mark.parametrize("n,offset", [((15, 15, 15), 0), ((20, 20, 20), 1), ((20, 20, 20), 0)])
+def test_korobov_collide_staging_paths(n, offset):
+    """The node kernel stages a cell's f into LDS by 16-byte DMA when f is
+    16-byte aligned and the grid's node count is even, else by 8-byte loads.
+    An odd grid (1,791 nodes: odd counts, every second cell 8-byte aligned
+    only) and an even grid whose f starts 8 bytes into its buffer take the
+    second path; all match the oracle, and the even grid's two paths give the
+    same bits."""
+    say("korobov staging paths", n, offset)
+    vm = 4.8
+    g = O.Grid(n, (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    M = 1 << 16
+    c.korobov_build(M, seed=5, s0=0.25)
+    info = c.korobov_info()
+    st = O.korobov_stencil(g, M, info["a"], 0.25)
+    cells = 3
+    rng = np.random.default_rng(12)
+    f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
+    buf = G.DeviceArray((cells * g.n_pnt + 1,))
+    host = np.zeros(cells * g.n_pnt + 1)
+    host[offset:offset + cells * g.n_pnt] = f.ravel()
+    buf.upload(host)
+    loss, Q = c.collide_korobov(_DeviceView(buf, offset), cells)
+    ol, oq = O.korobov_collide(g, st, f)
+    assert rel_l2(loss.get(), ol) < 1e-12
+    assert rel_l2(Q.get(), oq) < 1e-12
+    if g.n_pnt % 2 == 0:
+        l2, q2 = c.collide_korobov(G.DeviceArray.from_host(f), cells)
+        assert np.array_equal(q2.get(), Q.get())
+        assert np.array_equal(l2.get(), loss.get())
+
+
 def test_korobov_maxwellian_and_conservation():
     say("korobov maxwellian")
     vm = 4.8
