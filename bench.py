@@ -30,6 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector), datasheet
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp32": 157.3}  # MI355X dense matrix peaks, datasheet
 N_GRID = 32
 VMAX = 4.8
 M_NODES = 1 << 21
@@ -162,9 +164,22 @@ def batched_matrix(G, cells=8192, ref_cells=256, seed=SEED):
         Qm = Q.get()[:ref_cells]
         err = float(np.linalg.norm(Qm - Qr) / np.linalg.norm(Qr))
         errs = np.linalg.norm(Qm - Qr, axis=1) / np.linalg.norm(Qr, axis=1)
+        # matrix-core work per cell: the gain GEMMs (512 g x 512 n1 per parity
+        # class times its s columns: 31^3 over the 8 classes, 31 x 1,024 as
+        # executed, the y and z extents padded to 16) and the loss GEMM (4,096 x
+        # 4,096); the split mode issues three products per product
+        prods = 3 if mode == G.GEMM_BF16_SPLIT else 1
+        alg_cell = 2 * 512 * 512 * 31 ** 3 + 2 * 4096 * 4096
+        exe_cell = prods * (2 * 512 * 512 * 31 * 1024 + 2 * 4096 * 4096)
+        peak = MFMA_PEAK_TFLOPS["fp32" if mode == G.GEMM_F32 else "bf16"]
+        exe_tf = exe_cell * cells / best / 1e12
         out[name] = {"evals_per_s": round(cells / best, 1), "ms": round(1e3 * best, 2),
                      "rel_l2_Q_vs_fp64": err, "max_cell_rel_l2": float(errs.max()),
-                     "speedup_vs_fp64_gpu": round(cells / best / out["fp64_gpu_matrix"]["evals_per_s"], 1)}
+                     "speedup_vs_fp64_gpu": round(cells / best / out["fp64_gpu_matrix"]["evals_per_s"], 1),
+                     "mfma": {"alg_tflop": round(alg_cell * cells / 1e12, 1),
+                              "executed_tflop": round(exe_cell * cells / 1e12, 1),
+                              "executed_tflops": round(exe_tf, 1), "peak_tflops": peak,
+                              "frac": round(exe_tf / peak, 4)}}
         del nu, gain, Q
     return out
 
@@ -368,6 +383,14 @@ def main():
         # alternative fused kernel: 20-byte entry records + f + ptr + centres
         "kor_fused": ne * 20 + 4 * (n + 1) + 8 * n + 24 * n + 16 * n + 256 * 160,
     }
+    # FP64 operations per node of k_kor_node, counted from node_terms and
+    # kmath.h (an fma counts 2, a division or square root 1; frexp, rint,
+    # ldexp and conversions 0): with r != 0 (the log / exp branch) 5 products
+    # and quotients before the logarithm, kmt_log 24, r * log 1, kmt_exp 17,
+    # Pl * exp 1, w 2, D 2, 2wP 2 = 54; with r = 0, P, Pl, w, D, 2wP = 8.
+    # Every node with r != 0 contributes two r-role entries, the others none.
+    nz = (ne - 4 * nn) // 2
+    node_flop = 54 * nz + 8 * (nn - nz)
     traffic = profiled_traffic()
     roof = None
     kstats = {}
@@ -381,6 +404,10 @@ def main():
             if cnt:
                 kstats[k] = {"avg_us": 1e3 * ms / cnt, "launches": cnt,
                              "alg_bytes": alg[k], "gbs": alg[k] / (ms / cnt * 1e-3) / 1e9}
+        if "kor_node" in kstats:
+            tf = node_flop / (kstats["kor_node"]["avg_us"] * 1e-6) / 1e12
+            kstats["kor_node"].update({"fp64_flop": node_flop, "fp64_tflops": tf,
+                                       "fp64_frac": tf / FP64_PEAK_TFLOPS})
         dom = max(kstats, key=lambda k: kstats[k]["avg_us"])
         ach = kstats[dom]["gbs"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
