@@ -6,7 +6,10 @@
 // (determ.cu:146-160) -- i.e. velocity slot index / npx at cell index % npx.
 // The moments printed at the end are those of cell 0's collision integral
 // over its velocity nodes (the reference's print_moments, determ.cu:123-144,
-// mixes cells and velocities).
+// mixes cells and velocities), followed by cell 0's entropy H = sum f ln f d3v
+// and its rate sum Q ln f d3v (non-positive by the H theorem, up to the
+// discretisation), both for the f the last collision integral was evaluated
+// on.
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -46,7 +49,7 @@ static int run(int argc, char** argv)
     const size_t count = (size_t)opts.nxyz * opts.npx;
 
     // determ.cu:101-121: the same Maxwellian in every cell, [velocity][cell]
-    std::vector<float> h_f(count), h_dc(count), h_ic(count);
+    std::vector<float> h_f(count), h_dc(count), h_ic(count), h_flast(count);
     const float u1 = 1.75f, v1 = 0.25f;
     for (int i = 0, index = 0; i < opts.nx; i++)
         for (int j = 0; j < opts.ny; j++)
@@ -76,6 +79,7 @@ static int run(int argc, char** argv)
         gdt_memcpy_dtoh(&h_time, d_time, sizeof(float));
         std::cout << "STEP=" << step << ",  TIME=" << h_time << std::endl;
         if (step % out_step == 0) gdt_memcpy_dtoh(h_f.data(), d_f, count * sizeof(float));
+        if (step == opts.steps - 1) gdt_memcpy_dtoh(h_flast.data(), d_f, count * sizeof(float));
         evolve_colisions((float*)d_f, (float*)d_dc, (float*)d_ic, b, a, corr, opts);
         relax_f((float*)d_f, (float*)d_dc, (float*)d_ic, (float)opts.courant, (float*)d_time, opts);
     }
@@ -107,6 +111,19 @@ static int run(int argc, char** argv)
                 mom[4] += ci * (u * u + v * v + w * w) * vgrid.d3v;
             }
     std::printf("CELL 0: M0=%e, MU=%e, MV=%e, MW=%e, M2=%e\n", mom[0], mom[1], mom[2], mom[3], mom[4]);
+    if (opts.steps > 0) {
+        double H = 0.0, dH = 0.0;
+        for (int index = 0; index < opts.nxyz; index++) {
+            const size_t e = (size_t)index * opts.npx;  // cell 0
+            const double f = h_flast[e];
+            if (f > 0.0) {
+                const double q = -f * h_dc[e] + h_ic[e];
+                H += f * std::log(f) * vgrid.d3v;
+                dH += q * std::log(f) * vgrid.d3v;
+            }
+        }
+        std::printf("CELL 0: H=%e, DHDT=%e\n", H, dH);
+    }
     std::cout << "COLLISION INTEGRAL EVALUATION TOOK " << timer.elapsed() << " MS." << std::endl;
 
     gdt_free(d_f);

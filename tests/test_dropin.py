@@ -6,6 +6,8 @@ import subprocess
 import numpy as np
 import pytest
 
+from gpu_determ_amd import compare as CMP
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DETERM = os.path.join(ROOT, "gpu_determ_amd", "determ")
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -67,6 +69,18 @@ def test_driver_reproduces_golden(tmp_path):
     assert np.max(np.abs(mine[keep, 5] - gold["DC"][keep]) / gold["DC"][keep]) < 1e-5
     rel = np.abs(mine[:, 6] - gold["IC"]) / gold["IC"]
     assert np.max(rel[gold["race"] == 0]) < 1e-4 and np.max(rel) < 2e-4
+    # the comparator that replaces test.py, against the reference's
+    # true_results.out rebuilt from the golden rows (F and IC: the golden DC of
+    # cell 0 carries the reference's correction)
+    tr = tmp_path / "true_results.out"
+    idx = (gold["slot"] * 256 + gold["cell"]).astype(int)
+    with open(tr, "w") as fh:
+        for k, ix in enumerate(idx):
+            fh.write("%d %d %d %d %g %g %g %g %g\n" % (ix, ix // 256, (ix % 256) // 16, ix % 16, gold["F"][k],
+                                                        gold["DC"][k], gold["IC"][k], gold["CI"][k],
+                                                        gold["ACCURACY"][k]))
+    ok, _ = CMP.compare(tr, out, check=("F", "IC"))
+    assert ok
 
 
 @pytest.mark.gpu
@@ -79,7 +93,13 @@ def test_driver_corrected_run_conserves(tmp_path):
         conf.write_text("nx = 16\nny = 16\nnz = 16\nnpx = 64\ncorrection = %d\n" % corr)
         r = subprocess.run([DETERM, str(conf), str(tmp_path / "r.out")], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr
-        line = [l for l in r.stdout.splitlines() if l.startswith("CELL 0:")][0]
-        return np.array([float(t.split("=")[1].rstrip(",")) for t in line.split()[2:]])
-    mc, mu = run(1), run(0)
+        lines = [l for l in r.stdout.splitlines() if l.startswith("CELL 0:")]
+        vals = [np.array([float(t.split("=")[1].rstrip(",")) for t in l.split()[2:]]) for l in lines]
+        return vals[0], vals[1]
+    (mc, hc), (mu, hu) = run(1), run(0)
     assert abs(mc[0]) < 1e-3 * abs(mu[0]) and abs(mc[4]) < 1e-3 * abs(mu[4])
+    # entropy report (H, dH/dt) of the same f: the reference's initial f is a
+    # Maxwellian, whose ln f is a collision invariant, so dH/dt is a sum of
+    # Q's moments: corrected, it shrinks with them
+    assert np.all(np.isfinite(hc)) and np.all(np.isfinite(hu)) and hc[0] == hu[0]
+    assert abs(hc[1]) < abs(hu[1])
