@@ -8,16 +8,20 @@
 //   loss[a], loss[b] += 2 w P.
 //
 // Default path, three kernels per step:
-//   k_kor_node    node-parallel, f of the cell staged in LDS: D and 2wP of
-//                 every node, once, written contiguously (double2 per node);
+//   k_kor_node    node-parallel, f of the cell staged in LDS (16-byte LDS
+//                 DMA): D and 2wP of every node, once, written contiguously
+//                 (double2 per node); block b evaluates a slice of node group
+//                 b % KOR_GROUPS, on the XCD that gathers that group;
 //   k_kor_gather  destination-sorted reduction, XCD-aware: the nodes are
 //                 split into KOR_GROUPS contiguous ranges, each with its own
 //                 destination-sorted CSR of (node | role, coefficient -1 /
 //                 1-r / r) entries.  Workgroups are dealt to the XCDs round
 //                 robin, so block b serves group b % KOR_GROUPS and every XCD
 //                 gathers only its range's (D, 2wP) -- under 1 MB, resident in
-//                 that XCD's L2.  A 16-lane quarter-wave owns a destination,
-//                 sums its entries in a fixed order and writes the group's
+//                 that XCD's L2.  A block takes one tile of a group's
+//                 destinations: it parks every entry's term in LDS, then a
+//                 thread (or, for tiles of long rows, a quad) sums each
+//                 destination's terms in a fixed order and writes the group's
 //                 partial Q and loss; each block adds its destinations' moment
 //                 sums (linear, so partials add up);
 //   k_kor_apply   sums the group partials in group order, solves the 5x5
