@@ -116,8 +116,8 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
     out = {"cells": cells, "grids": {"%d^3" % n: int(np.sum(pick == i)) for i, n in enumerate(sizes)},
            "korobov_M": 1 << 20, "steps": steps, "evals_per_s": round(cells * steps / el, 1),
            "ms_per_step": round(1e3 * el / steps, 3), "eager_evals_per_s": round(cells * steps / el_eager, 1),
-           "tuned": {"seg_cells": seg, "nodes_per_thread": q,
-                                                                "best_eval_ms": round(tune_ms, 3)},
+           "tuned": {"seg_cells": seg, "nodes_per_thread": q, "best_eval_ms": round(tune_ms, 3),
+                     "gather_tiles_per_group": {"%d^3" % n: c.korobov_tiles()["tiles"] for n, c in zip(sizes, ctxs)}},
            "setup_s": round(setup_s, 2), "tune_s": round(tune_s, 2)}
     b.close()
     return out

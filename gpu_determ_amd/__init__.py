@@ -44,7 +44,7 @@ ABI_SYMBOLS = (
     "gdt_gemm_nt", "gdt_split_f64", "gdt_matrix_mfma",
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
-    "gdt_init_mixture", "gdt_korobov_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
+    "gdt_init_mixture", "gdt_korobov_tiles", "gdt_korobov_set_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
 )
 
 _lib = None
@@ -126,6 +126,7 @@ def lib():
         "gdt_surrogate_info": [vp, ip, ip, ip, C.POINTER(C.c_long)],
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
         "gdt_korobov_tiles": [vp, ip, ip, ip],
+        "gdt_korobov_set_tiles": [vp, C.c_int],
         "gdt_kmath_pow": [vp, vp, vp, vp, C.c_long],
         "gdt_kmath_tables": [vp],
     }
@@ -262,6 +263,11 @@ class Context:
         g, t, cap = C.c_int(), C.c_int(), C.c_int()
         check(lib().gdt_korobov_tiles(self.h, C.byref(g), C.byref(t), C.byref(cap)))
         return {"groups": g.value, "tiles": t.value, "cap": cap.value}
+
+    def korobov_set_tiles(self, tiles_max):
+        """Limit of gather tiles per node group (default 64); rebuilds the
+        tiles of a built stencil."""
+        check(lib().gdt_korobov_set_tiles(self.h, int(tiles_max)))
 
     def kmath_pow(self, x, r):
         """x**r through the node terms' table-driven log / exp (FP64 device

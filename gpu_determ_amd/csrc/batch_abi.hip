@@ -21,6 +21,7 @@ struct gdt_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<gdt_context *> cls;
+    std::vector<unsigned> cls_gen;  // the classes' gather layouts the work lists were built for
     std::vector<double *> f, Q;
     std::vector<int> cells;
     int lds_node = 0, lds_gather = 0, cu = 256;
@@ -189,6 +190,15 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
     }
     b->seg_cells = seg_cells;
     b->node_q = node_q;
+    // dynamic LDS of the launches, from the classes' current layouts (a
+    // re-tiling changes the gather's entries per pass)
+    b->lds_node = b->lds_gather = 0;
+    b->cls_gen.resize(ncls);
+    for (int s = 0; s < ncls; s++) {
+        b->lds_node = std::max(b->lds_node, kor_node_lds(b->cls[s]));
+        b->lds_gather = std::max(b->lds_gather, (int)sizeof(double2) * b->cls[s]->kcap);
+        b->cls_gen[s] = b->cls[s]->kor_gen;
+    }
     return GDT_OK;
 }
 
@@ -216,7 +226,6 @@ int gdt_batch_create(gdt_batch_handle *out, int nclass, const gdt_handle *classe
     GDT_CHECK(out && nclass > 0 && classes && f && cells, GDT_E_ARG, "gdt_batch_create: bad arguments");
     *out = nullptr;
     gdt_batch *b = new gdt_batch();
-    int lds_node = 0, lds_gather = 0;
     for (int s = 0; s < nclass; s++) {
         gdt_context *h = classes[s];
         if (!h || !h->have_kor || !h->have_stream || !f[s] || cells[s] < 0) {
@@ -232,12 +241,8 @@ int gdt_batch_create(gdt_batch_handle *out, int nclass, const gdt_handle *classe
         b->cls.push_back(h);
         b->f.push_back(f[s]);
         b->cells.push_back(cells[s]);
-        lds_node = std::max(lds_node, kor_node_lds(h));
-        lds_gather = std::max(lds_gather, (int)sizeof(double2) * h->kcap);
     }
     b->device = classes[0]->device;
-    b->lds_node = lds_node;
-    b->lds_gather = lds_gather;
     if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         (void)hipGetLastError();
@@ -248,7 +253,7 @@ int gdt_batch_create(gdt_batch_handle *out, int nclass, const gdt_handle *classe
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
         b->cu = prop.multiProcessorCount;
-    int rc = batch_set_lds(lds_node, lds_gather);
+    int rc = batch_set_lds();
     if (rc) {
         gdt_batch_destroy(b);
         return rc;
@@ -280,6 +285,17 @@ int gdt_batch_set_output(gdt_batch_handle b, double *const *Q)
     return GDT_OK;
 }
 
+// A class re-tiled or rebuilt after the work lists were made would leave them
+// pointing at its old layout.
+static int batch_check_classes(const gdt_batch *b)
+{
+    for (size_t s = 0; s < b->cls.size() && s < b->cls_gen.size(); s++)
+        GDT_CHECK(b->cls[s]->kor_gen == b->cls_gen[s], GDT_E_STATE,
+                  "gdt_batch: class %zu changed its stencil or tiles after the batch was configured "
+                  "(call gdt_batch_config again)", s);
+    return GDT_OK;
+}
+
 int gdt_batch_config(gdt_batch_handle b, int seg_cells, int node_q)
 {
     B_CHECK(b);
@@ -298,22 +314,46 @@ int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node
     for (int k : {64, 256, 1024})
         if (k < total) segs.push_back(k);
     if (total <= 1024) segs.push_back(total);
+    // the classes' gather tile limit first, at the largest segment and 32
+    // nodes per thread (the tile count sets the moment partials per cell and
+    // the per-tile work: fewer, larger tiles pay off when many cells share the
+    // GPU), then (segment, nodes per thread) at the chosen limit
+    auto time_step = [&](float &t) -> int {
+        t = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {  // evaluation only (dt = 0): f is not changed
+            GDT_HIP(hipEventRecord(b->ev0, b->stream));
+            if (int rc = batch_step(b, 0.0, flags)) return rc;
+            GDT_HIP(hipEventRecord(b->ev1, b->stream));
+            GDT_HIP(hipEventSynchronize(b->ev1));
+            float ms = 0.0f;
+            GDT_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            t = std::min(t, ms);
+        }
+        return GDT_OK;
+    };
+    float best_t = 1e30f;
+    int bt = KOR_TILES_MAX;
+    for (int tl : {32, 64, 128}) {
+        for (gdt_context *c : b->cls)
+            if (int rc = gdt_korobov_set_tiles(c, tl)) return rc;
+        if (int rc = batch_configure(b, segs.back(), 32)) return rc;
+        float t;
+        if (int rc = time_step(t)) return rc;
+        if (t < best_t) {
+            best_t = t;
+            bt = tl;
+        }
+    }
+    for (gdt_context *c : b->cls)
+        if (int rc = gdt_korobov_set_tiles(c, bt)) return rc;
     float best = 1e30f;
     int bs = segs.back(), bq = 2;
     for (int k : segs)
         for (int q : {2, 8, 32, 128}) {
             int rc = batch_configure(b, k, q);
             if (rc) return rc;
-            float t = 1e30f;
-            for (int rep = 0; rep < 2; rep++) {  // evaluation only (dt = 0): f is not changed
-                GDT_HIP(hipEventRecord(b->ev0, b->stream));
-                if ((rc = batch_step(b, 0.0, flags))) return rc;
-                GDT_HIP(hipEventRecord(b->ev1, b->stream));
-                GDT_HIP(hipEventSynchronize(b->ev1));
-                float ms = 0.0f;
-                GDT_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-                t = std::min(t, ms);
-            }
+            float t;
+            if ((rc = time_step(t))) return rc;
             if (t < best) {
                 best = t;
                 bs = k;
@@ -336,6 +376,7 @@ int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags)
         int rc = batch_configure(b, 256, 2);
         if (rc) return rc;
     }
+    if (int rc = batch_check_classes(b)) return rc;
     if (!(flags & GDT_FLAG_GRAPH)) {
         for (int s = 0; s < steps; s++) {
             int rc = batch_step(b, dt, flags);

@@ -448,6 +448,51 @@ uint32_t gdt_korobov_pick(uint32_t M, uint64_t seed, int ncand, double *p2)
     return host_korobov_pick(M, seed, ncand, p2);
 }
 
+// Gather tiles of the stencil from the host copy of the group row pointers:
+// per group, contiguous destination ranges of at most KOR_TILE_DST
+// destinations and about 1/NT of the group's entries; NT is the largest count
+// (<= h->ktiles_max) for which every group needs at most h->ktiles_max tiles
+// (G * ktiles_max blocks per cell).  Tiles hold whole destinations, so the
+// per-destination sums do not depend on the tiling; the moment partials (one
+// per tile) do, in their last bits.
+static int kor_make_tiles(gdt_context *h)
+{
+    const int G = KOR_NGROUPS, np = h->n_pnt, tmax = h->ktiles_max;
+    const std::vector<int32_t> &gp = h->h_gp;
+    std::vector<int4> tiles;
+    int ntile = 0, maxe = 0;
+    for (int NT = tmax; NT >= 1; NT--) {
+        std::vector<std::vector<int4>> per(G);
+        ntile = 0;
+        maxe = 0;
+        for (int g = 0; g < G; g++) {
+            const int32_t *pg = &gp[(size_t)g * (np + 1)];
+            const long quota = std::max(1L, ((long)pg[np] - pg[0] + NT - 1) / NT);
+            int d = 0;
+            while (d < np) {
+                int e = d;
+                while (e < np && e - d < KOR_TILE_DST && (e == d || (long)pg[e] - pg[d] < quota)) e++;
+                per[g].push_back(make_int4(d, e, pg[d], pg[e]));
+                maxe = std::max(maxe, pg[e] - pg[d]);
+                d = e;
+            }
+            ntile = std::max(ntile, (int)per[g].size());
+        }
+        if (ntile <= tmax || NT == 1) {
+            tiles.assign((size_t)G * ntile, make_int4(0, 0, 0, 0));
+            for (int g = 0; g < G; g++)
+                for (size_t k = 0; k < per[g].size(); k++) tiles[(size_t)g * ntile + k] = per[g][k];
+            break;
+        }
+    }
+    GDT_CHECK(h->ktile.ensure(sizeof(int4) * tiles.size()) == 0, GDT_E_NOMEM, "gather tile allocation failed");
+    GDT_HIP(hipMemcpy(h->ktile.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+    h->kntile = ntile;
+    h->kcap = std::max(64, std::min(maxe, KOR_TILE_CAP));
+    h->kor_gen++;
+    return GDT_OK;
+}
+
 int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, double s0)
 {
     H_CHECK(h);
@@ -534,7 +579,7 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
     h->nn = nn;  // read by the wave partition below
     h->nent = ne;
     // packed records (16-bit node indices, the f of a cell in LDS)
-    bool stream_ok = np <= 65535 && (size_t)np * sizeof(double) <= 150 * 1024 && np < (1 << 18);
+    bool stream_ok = np <= KOR_NPTS_MAX;
     for (long t = 0; t < nn && stream_ok; t++)
         stream_ok = std::abs(cint[3 * t]) < 32768 && std::abs(cint[3 * t + 1]) < 32768 && cint[3 * t + 2] < 4096;
     GDT_CHECK(stream_ok, GDT_E_ARG,
@@ -618,44 +663,14 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 gent[2 * (size_t)pos + 1] = n4[4 * (size_t)p + 3];
             }
         }
-        // gather tiles: per group, contiguous destination ranges of at most
-        // KOR_TILE_DST destinations and about 1/NT of the group's entries; NT
-        // is the largest count (<= KOR_TILES_MAX) for which every group needs at
-        // most KOR_TILES_MAX tiles (G * KOR_TILES_MAX blocks in one round)
-        std::vector<int4> tiles;
-        int ntile = 0, maxe = 0;
-        for (int NT = KOR_TILES_MAX; NT >= 1; NT--) {
-            std::vector<std::vector<int4>> per(G);
-            ntile = 0;
-            maxe = 0;
-            for (int g = 0; g < G; g++) {
-                const int32_t *pg = &gp[(size_t)g * (np + 1)];
-                const long quota = std::max(1L, ((long)pg[np] - pg[0] + NT - 1) / NT);
-                int d = 0;
-                while (d < np) {
-                    int e = d;
-                    while (e < np && e - d < KOR_TILE_DST && (e == d || (long)pg[e] - pg[d] < quota)) e++;
-                    per[g].push_back(make_int4(d, e, pg[d], pg[e]));
-                    maxe = std::max(maxe, pg[e] - pg[d]);
-                    d = e;
-                }
-                ntile = std::max(ntile, (int)per[g].size());
-            }
-            if (ntile <= KOR_TILES_MAX || NT == 1) {
-                tiles.assign((size_t)G * ntile, make_int4(0, 0, 0, 0));
-                for (int g = 0; g < G; g++)
-                    for (size_t k = 0; k < per[g].size(); k++) tiles[(size_t)g * ntile + k] = per[g][k];
-                break;
-            }
-        }
-        GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * gp.size()) == 0 && h->kgent.ensure(sizeof(uint32_t) * 2 * ne) == 0 &&
-                      h->ktile.ensure(sizeof(int4) * tiles.size()) == 0,
+        h->h_gp.swap(gp);
+        GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * h->h_gp.size()) == 0 &&
+                      h->kgent.ensure(sizeof(uint32_t) * 2 * ne) == 0,
                   GDT_E_NOMEM, "gdt_korobov_build: group CSR allocation failed");
-        GDT_HIP(hipMemcpy(h->kgptr.p, gp.data(), sizeof(int32_t) * gp.size(), hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kgptr.p, h->h_gp.data(), sizeof(int32_t) * h->h_gp.size(), hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->kgent.p, gent.data(), sizeof(uint32_t) * 2 * ne, hipMemcpyHostToDevice));
-        GDT_HIP(hipMemcpy(h->ktile.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
-        h->kntile = ntile;
-        h->kcap = std::max(64, std::min(maxe, KOR_TILE_CAP));
+        int rc1 = kor_make_tiles(h);
+        if (rc1) return rc1;
         int rc2 = kor_setup(h, 1);
         if (rc2) return rc2;
         h->have_stream = true;
@@ -1079,4 +1094,22 @@ int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap)
     if (tiles) *tiles = h->kntile;
     if (cap) *cap = h->kcap;
     return GDT_OK;
+}
+
+int gdt_korobov_set_tiles(gdt_handle h, int tiles_max)
+{
+    H_CHECK(h);
+    GDT_CHECK(tiles_max >= 1 && tiles_max <= KOR_TILES_LIMIT, GDT_E_ARG,
+              "gdt_korobov_set_tiles: %d tiles per group (1..%d)", tiles_max, KOR_TILES_LIMIT);
+    h->ktiles_max = tiles_max;
+    if (!h->have_kor) return GDT_OK;  // applies to the next build
+    GDT_HIP(hipSetDevice(h->device));
+    GDT_HIP(hipStreamSynchronize(h->stream));  // no step in flight reads the old tiles
+    int rc = kor_make_tiles(h);
+    if (rc) return rc;
+    if (h->graph_exec) {  // captured launches carry the old grid
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    return kor_setup(h, 1);
 }

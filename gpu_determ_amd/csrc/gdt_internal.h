@@ -30,6 +30,14 @@ void gdt_set_error(int code, const char *fmt, ...);
         }                                         \
     } while (0)
 
+// Korobov gather layout
+#define KOR_NGROUPS 8     // node groups of the gather: one per XCD
+#define KOR_TILE_DST 512  // destinations per gather tile: one per thread
+#define KOR_TILE_CAP 4864 // entries per gather pass (76 KB of LDS at most)
+#define KOR_TILES_MAX 64  // default gather tiles per group: G * 64 blocks, two per CU, one round
+#define KOR_TILES_LIMIT 1024
+#define KOR_NPTS_MAX 19200 // grid nodes the packed node records and the LDS copy of f admit
+
 // Device buffer owned by a context.
 struct DevBuf {
     void *p = nullptr;
@@ -77,6 +85,9 @@ struct gdt_context {
     DevBuf ktile;                 // gather tiles (d0, d1, e0, e1), [group][kntile]
     DevBuf kmath;                 // log / exp tables of the node terms (KMT_SIZE doubles)
     int kntile = 0, kcap = 0;     // tiles per group, LDS entries per gather pass
+    int ktiles_max = KOR_TILES_MAX;  // limit of tiles per group (gdt_korobov_set_tiles)
+    std::vector<int32_t> h_gp;    // host copy of the group CSR row pointers [G][n+1]
+    unsigned kor_gen = 0;         // bumped whenever the stencil's gather layout changes
     std::vector<int32_t> h_kptr;  // host copy of the CSR row pointers
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
     int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
@@ -207,7 +218,7 @@ int launch_batch_node(hipStream_t s, const KorClassDev *cls, const int4 *items, 
 int launch_batch_gather(hipStream_t s, const KorClassDev *cls, const int4 *items, int nitems, int blocks, int lds);
 int launch_batch_apply(hipStream_t s, bool correct, const KorClassDev *cls, const int4 *items, int nitems, int blocks,
                        double dt);
-int batch_set_lds(int lds_node, int lds_gather);
+int batch_set_lds();
 
 // ---- Korobov kernels (korobov_kernels.hip) --------------------------------
 int kor_blocks(gdt_context *h, int cells);
@@ -216,10 +227,6 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells);
 int kor_gather_blocks(gdt_context *h, int cells);  // blocks launched per cell (one per tile)
 int kor_gather_parts(gdt_context *h);              // moment partials per cell (one per tile)
 int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Qpart, double *mpart, int cells);
-#define KOR_NGROUPS 8     // node groups of the gather: one per XCD
-#define KOR_TILE_DST 512  // destinations per gather tile: one per thread
-#define KOR_TILE_CAP 4864 // entries per gather pass (76 KB of LDS at most)
-#define KOR_TILES_MAX 128 // gather tiles per group: G * 128 blocks, four per CU, one round
 // First node position of gather group g (node p is in group p * G / nn, so
 // group g is [kor_group_begin(g), kor_group_begin(g + 1)))
 __host__ __device__ constexpr long kor_group_begin(int g, long nn)

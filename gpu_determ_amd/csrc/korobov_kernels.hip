@@ -261,9 +261,11 @@ __device__ __forceinline__ void zero_empty(int ga, int gb, const int32_t *ptr, d
 //            their entries of the pass in entry order (fixed order:
 //            reproducible sums).
 // The destinations' row pointers and coordinates are loaded before phase 1.
-// Tiles of about 2,300 entries (37 KB of LDS) and batches of TILE_BATCH
-// entries per thread (one batch per tile; 112 VGPRs): the G * 128 blocks of
-// a cell run in one round, four per CU.
+// By default 64 tiles per group: at 32^3 and 2^21 nodes about 4,600 entries
+// per tile (74 KB of LDS, two batches of TILE_BATCH entries per thread; 112
+// VGPRs), so the G * 64 blocks of a cell run in one round, two per CU.  (128
+// smaller tiles, four per CU, gather a little faster but give the apply twice
+// the moment partials: the step is 0.3 us slower.)
 #define TILE_THREADS 256
 #define TILE_WAVES (TILE_THREADS / WAVE)
 #define TILE_DPT (KOR_TILE_DST / TILE_THREADS)  // destinations per thread
@@ -692,8 +694,14 @@ __global__ __launch_bounds__(APPLY_THREADS) void k_batch_apply(const KorClassDev
     }
 }
 
-int batch_set_lds(int lds_node, int lds_gather)
+// The work-list kernels may take the largest dynamic LDS any class can need
+// (f of KOR_NPTS_MAX nodes and the tables; KOR_TILE_CAP parked terms): the
+// attribute is a per-kernel limit shared by every batch of the process, and
+// each launch passes its own batch's size.
+int batch_set_lds()
 {
+    const int lds_node = (int)sizeof(double) * (kor_fpad(KOR_NPTS_MAX) + KMT_SIZE);
+    const int lds_gather = (int)sizeof(double2) * KOR_TILE_CAP;
     GDT_HIP(hipFuncSetAttribute((const void *)k_batch_node, hipFuncAttributeMaxDynamicSharedMemorySize, lds_node));
     GDT_HIP(hipFuncSetAttribute((const void *)k_batch_gather, hipFuncAttributeMaxDynamicSharedMemorySize, lds_gather));
     return GDT_OK;
@@ -807,7 +815,7 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
 
 int kor_gather_parts(gdt_context *h) { return KOR_GROUPS * h->kntile; }
 
-// one block per tile (the G * KOR_TILES_MAX tiles of a cell run in one round)
+// one block per tile (G * at most ktiles_max tiles per cell)
 int kor_gather_blocks(gdt_context *h, int cells)
 {
     (void)cells;

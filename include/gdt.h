@@ -123,6 +123,15 @@ int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32
 /* Gather layout of the stencil: node groups (one per XCD), destination
  * tiles per group (one moment partial each), LDS entries per gather pass. */
 int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
+/* Limit of gather tiles per node group (default 64, 1..1024); no reference
+ * counterpart (a launch-shape setting of the gather).  Fewer, larger tiles
+ * give the apply fewer moment partials and the gather less per-tile work;
+ * more tiles put more blocks in flight.  On a built stencil the tiles are
+ * rebuilt at once (after the context's stream drains).  Per-destination sums
+ * do not depend on the tiling; the moment sums do, in their last bits.  A
+ * batch holding this context must be configured again (gdt_batch_config);
+ * gdt_batch_tune sets it on the batch's classes. */
+int gdt_korobov_set_tiles(gdt_handle h, int tiles_max);
 /* Accuracy hook, no reference counterpart: out[i] = x[i]^r[i] as the node
  * terms compute it (table-driven FP64 log and exp); device pointers, x > 0;
  * synchronous. */
@@ -239,8 +248,11 @@ int gdt_batch_destroy(gdt_batch_handle b);
 int gdt_batch_set_output(gdt_batch_handle b, double *const *Q);
 /* Explicit launch shape: cells per segment, nodes per thread of the node kernel (>= 2). */
 int gdt_batch_config(gdt_batch_handle b, int seg_cells, int node_q);
-/* Autotuner: times every (segment size, nodes per thread) candidate on the
- * batch's cells (evaluation only, f unchanged) and keeps the fastest. */
+/* Autotuner: on the batch's cells (evaluation only, f unchanged) times the
+ * classes' gather tile limit (32, 64, 128; set on every class with
+ * gdt_korobov_set_tiles, so single runs of those contexts keep matching the
+ * batch bitwise), then every (segment size, nodes per thread) candidate, and
+ * keeps the fastest. */
 int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node_q, double *best_ms);
 int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags);
 int gdt_batch_synchronize(gdt_batch_handle b);

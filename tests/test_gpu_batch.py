@@ -101,3 +101,41 @@ def test_batch_eager_equals_graph(classes):
         b.close()
     for a, c in zip(*outs):
         assert np.array_equal(a, c)
+
+
+def test_korobov_set_tiles_and_batch_layout_check():
+    """gdt_korobov_set_tiles: per-destination sums are the same bits for any
+    tile limit, the corrected Q differs only in the moment sums' last bits,
+    and returning to the default restores the default's bits.  A batch built
+    before a re-tiling refuses to run until it is configured again, and then
+    matches single runs at the new tiling bitwise."""
+    print("korobov set tiles", flush=True)
+    ctx = make_classes((16,))[0]
+    rng = np.random.default_rng(21)
+    f0 = cell_fs(ctx, 2, rng)
+    fd = G.DeviceArray.from_host(f0)
+    l1, q1 = ctx.collide_korobov(fd, 2)
+    _, _, Qc1 = ctx.evaluate(G.METHOD_KOROBOV, fd, 2, G.FLAG_CORRECT)
+    ctx.korobov_set_tiles(8)
+    assert ctx.korobov_tiles()["tiles"] <= 8
+    l2, q2 = ctx.collide_korobov(fd, 2)
+    _, _, Qc2 = ctx.evaluate(G.METHOD_KOROBOV, fd, 2, G.FLAG_CORRECT)
+    assert np.array_equal(q1.get(), q2.get()) and np.array_equal(l1.get(), l2.get())
+    a, b_ = Qc1.get(), Qc2.get()
+    assert np.max(np.abs(a - b_)) <= 1e-12 * np.max(np.abs(a))
+    # a batch configured at 8 tiles, then the class re-tiled: refused until reconfigured
+    bd = G.DeviceArray.from_host(f0)
+    bt = G.Batch([ctx], [bd])
+    bt.config(2, 2)
+    ctx.korobov_set_tiles(64)  # the default limit
+    with pytest.raises(RuntimeError):
+        bt.run(DT, 1, G.FLAG_CORRECT)
+    bt.config(2, 2)
+    bt.run(DT, 1, G.FLAG_CORRECT)
+    bt.sync()
+    ref = single_runs(ctx, f0, 1, G.FLAG_CORRECT)
+    assert np.array_equal(bd.get(), ref)
+    _, _, Qc3 = ctx.evaluate(G.METHOD_KOROBOV, fd, 2, G.FLAG_CORRECT)
+    assert np.array_equal(Qc3.get(), a)
+    with pytest.raises(RuntimeError):
+        ctx.korobov_set_tiles(0)
