@@ -10,7 +10,7 @@ every step.  One step = one corrected Q(f,f) evaluation + the Euler update:
 three kernels launched onto the stream (--graph: replayed as a hipGraph).
 Metric: Q(f,f) evaluations per second.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--graph]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 300] [--graph]
 
 Prints ONE JSON line.  The roofline object reports the dominant kernel's
 algorithmic bytes over its hipEvent-timed average duration (a separate
@@ -320,13 +320,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=300,
+                    help="untimed steps first (300, about 8 ms, bring the GPU to its steady state: "
+                         "27.0 against 27.6 us per step after 5)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roofline", action="store_true", help="skip the instrumented kernel-timing pass")
     ap.add_argument("--fused", action="store_true", help="per-entry recomputation path (GDT_FLAG_FUSED)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the steps as a hipGraph (GDT_FLAG_GRAPH) instead of launching them on the stream")
-    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch extension figure")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the extension legs (uneven batch, matrix cores, surrogate)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
