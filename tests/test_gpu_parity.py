@@ -281,9 +281,9 @@ def test_korobov_fused_equals_node_gather():
     run-to-run reproducible bit for bit."""
     say("korobov fused vs node+gather")
     vm = 4.8
-    # 3 cells: the fused grid has 256 / 3 = 85 blocks, an odd count of moment
-    # partials (the apply's one-at-a-time reduction path)
-    for n, M, cells in (((20, 20, 20), 1 << 18, 2), ((32, 32, 32), 1 << 19, 3)):
+    # 37 cells: the fused grid has ceil(256 / 37) = 7 blocks per cell, an odd
+    # count of moment partials (the apply's one-at-a-time reduction path)
+    for n, M, cells in (((20, 20, 20), 1 << 18, 2), ((32, 32, 32), 1 << 19, 3), ((16, 16, 16), 1 << 16, 37)):
         g = O.Grid(n, (-vm,) * 3, (vm,) * 3, vm)
         c = ctx_for(g)
         c.korobov_build(M, seed=21, s0=0.25)
