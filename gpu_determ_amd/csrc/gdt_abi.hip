@@ -596,14 +596,18 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
             o4[3] = ((uint32_t)cint[3 * t] & 0xFFFFu) | (((uint32_t)cint[3 * t + 1] & 0xFFFFu) << 16);
         };
         // Node order of the packed records (node values and gather): sorted
-        // by the lambda destination, lattice order among equals.  The entries
-        // a destination receives as lambda then come from one contiguous run
-        // of node values (coalesced loads in the gather), and its lambda+s
-        // entries from the runs of its neighbours.
+        // by the lambda destination, then by the lambda+s destination,
+        // lattice order among equals.  The entries a destination receives as
+        // lambda then come from one contiguous run of node values (coalesced
+        // loads in the gather), and those it receives as lambda+s from one
+        // contiguous sub-run of each neighbour's run (step 26.5 -> 26.3 us
+        // against lambda alone).
         std::vector<uint32_t> order(nn);
         for (long t = 0; t < nn; t++) order[t] = (uint32_t)t;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](uint32_t x, uint32_t y) { return cidx6[6 * (size_t)x + 2] < cidx6[6 * (size_t)y + 2]; });
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            const int32_t lx = cidx6[6 * (size_t)x + 2], ly = cidx6[6 * (size_t)y + 2];
+            return lx != ly ? lx < ly : cidx6[6 * (size_t)x + 4] < cidx6[6 * (size_t)y + 4];
+        });
         std::vector<uint32_t> n4(4 * (size_t)nn), n1(nn);
         for (long p = 0; p < nn; p++) {
             pack(order[p], &n4[4 * p]);
