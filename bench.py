@@ -184,6 +184,32 @@ def batched_matrix(G, cells=8192, ref_cells=256, seed=SEED):
     return out
 
 
+def transport_step(G, shape=(16, 16, 16), steps=20, seed=SEED):
+    """Spatial transport (first-order upwind, periodic) of f on the 24^3
+    velocity grid over a box of `shape` cells: HBM rate of its algorithmic
+    bytes (f in + f out, 16 B per cell and velocity node)."""
+    rng = np.random.default_rng(seed + 6)
+    ctx = G.Context((24,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+    cells = int(np.prod(shape))
+    f = np.stack([mixture(ctx.xi, rng, comps=1) for _ in range(cells)])
+    a, b = G.DeviceArray.from_host(f), G.DeviceArray((cells, ctx.n_pnt))
+    h = (1.0, 1.0, 1.0)
+    dt = 0.9 / np.max(np.abs(ctx.xi).sum(axis=1))
+    for _ in range(3):
+        ctx.transport(a, shape, h, dt, out=b)
+        a, b = b, a
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.transport(a, shape, h, dt, out=b)
+        a, b = b, a
+    ctx.sync()
+    el = (time.perf_counter() - t0) / steps
+    alg = 16.0 * cells * ctx.n_pnt
+    return {"grid": "24^3", "nodes": ctx.n_pnt, "cells": "x".join(str(x) for x in shape), "us_per_step": round(1e6 * el, 2),
+            "alg_bytes": int(alg), "gbs": round(alg / el / 1e9, 1), "frac_of_hbm": round(alg / el / 1e9 / HBM_PEAK_GBS, 3)}
+
+
 def surrogate_form(G, train_cells=200000, infer_cells=16384, epochs=10, threshold=1e-4, seed=SEED):
     """Learned surrogate on 24^3 (vmax 4.8, sphere cut, 7,208 nodes): trained
     on `train_cells` (f, Q) pairs from the plain solver (Korobov 2^20,
@@ -436,6 +462,7 @@ def main():
         ext["uneven_batch"] = uneven_batch(G)
         ext["batched_matrix_mfma"] = batched_matrix(G)
         ext["surrogate"] = surrogate_form(G)
+        ext["transport"] = transport_step(G)
 
     value = args.steps / elapsed
     out = {

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libgdt.so")
 
 LAYOUT_CELL_MAJOR = 0
 LAYOUT_NODE_MAJOR = 1
+BC_PERIODIC = 0
+BC_ZERO_GRADIENT = 1
 TABLES_FP64 = 0
 TABLES_REFERENCE = 1
 METHOD_MATRIX = 0
@@ -45,6 +47,7 @@ ABI_SYMBOLS = (
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
     "gdt_init_mixture", "gdt_korobov_tiles", "gdt_korobov_set_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
+    "gdt_transport",
 )
 
 _lib = None
@@ -127,6 +130,7 @@ def lib():
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
         "gdt_korobov_tiles": [vp, ip, ip, ip],
         "gdt_korobov_set_tiles": [vp, C.c_int],
+        "gdt_transport": [vp, vp, vp, C.c_int, C.c_int, C.c_int, dp, C.c_double, ip],
         "gdt_kmath_pow": [vp, vp, vp, vp, C.c_long],
         "gdt_kmath_tables": [vp],
     }
@@ -326,6 +330,18 @@ class Context:
 
     def run_steps(self, method, f, cells, dt, steps, flags=FLAG_CORRECT):
         check(lib().gdt_run_steps(self.h, int(method), f.ptr, int(cells), float(dt), int(steps), int(flags)))
+
+    def transport(self, f, shape, spacing, dt, bc=(BC_PERIODIC,) * 3, out=None):
+        """One first-order upwind transport step of f [cells][n_pnt] over a box
+        of shape (nx, ny, nz) cells (cell (ix ny + iy) nz + iz); returns f_out
+        (a new DeviceArray unless `out` is given; it must not be f)."""
+        nx, ny, nz = (int(x) for x in shape)
+        if out is None:
+            out = DeviceArray((nx * ny * nz, self.n_pnt))
+        sp = (C.c_double * 3)(*[float(x) for x in spacing])
+        b = (C.c_int * 3)(*[int(x) for x in bc])
+        check(lib().gdt_transport(self.h, f.ptr, out.ptr, nx, ny, nz, sp, float(dt), b))
+        return out
 
     def init_mixture(self, params):
         """f [cells][n_pnt] (DeviceArray) from params [cells][comps][5] = (n, T, ux, uy, uz)."""

@@ -260,6 +260,21 @@ int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags);
 int gdt_batch_synchronize(gdt_batch_handle b);
 int gdt_batch_info(gdt_batch_handle b, int *seg_cells, int *node_q, int *segments);
 
+/* ---- spatial transport (no reference counterpart) ------------------------ */
+#define GDT_BC_PERIODIC 0
+#define GDT_BC_ZERO_GRADIENT 1 /* a missing neighbour takes the cell's own value */
+/* One explicit first-order upwind step of df/dt + xi . grad_x f = 0 over a box
+ * of nx x ny x nz cells (cell c = (ix ny + iy) nz + iz), f and f_out FP64
+ * [cell][node] device arrays that must not overlap; spacing[3] the cell
+ * sizes, bc[3] the boundary of each axis.  Per axis a, k = dt xi_a / h_a and
+ * f_out = f - sum_a k (f - f[c - e_a]) for k > 0, k (f[c + e_a] - f)
+ * otherwise (axes x, y, z in that order).  Rejects dt with
+ * max_v sum_a |k_a| > 1.  Periodic boundaries conserve every velocity's cell
+ * sum.  Asynchronous on the handle's stream.  This is the step a kinetic
+ * solver alternates with the collision operator (gdt_run_steps). */
+int gdt_transport(gdt_handle h, const double *f, double *f_out, int nx, int ny, int nz, const double *spacing,
+                  double dt, const int *bc);
+
 /* ---- initial conditions ---------------------------------------------------- */
 /* Sums of Maxwellians on the handle's grid (the reference's init_f is one
  * component, determ.cu:101-121): f[c][v] = sum_k n (2 pi T)^(-3/2)

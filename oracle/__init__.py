@@ -250,3 +250,30 @@ def korobov_collide(g, st, f_cmp):
 def courant_dt(nu, courant):
     nu = np.ascontiguousarray(nu, dtype=np.float64).ravel()
     return lib().orc_courant_dt(nu, nu.size, float(courant))
+
+
+def transport_upwind(f, xi, shape, spacing, dt, bc=(0, 0, 0)):
+    """One first-order upwind step of df/dt + xi . grad_x f = 0 (no reference
+    counterpart; the spatial step beside the collision operator).
+
+    f [cells][n] over a box of shape (nx, ny, nz) cells, cell (ix ny + iy) nz +
+    iz; xi [n][3].  Per axis a in the order x, y, z: k = (dt / h_a) xi_a,
+    D = f - f[behind] for k > 0, f[ahead] - f otherwise, f <- f - k D (each a
+    separate rounding, as the kernel computes it).  bc[a] = 0 periodic, 1
+    zero-gradient (a missing neighbour is the cell itself)."""
+    nx, ny, nz = shape
+    n = xi.shape[0]
+    f4 = np.asarray(f, dtype=np.float64).reshape(nx, ny, nz, n)
+    acc = f4.copy()
+    for a in range(3):
+        k = (dt / spacing[a]) * xi[:, a]
+        if bc[a] == 0:
+            back = np.roll(f4, 1, axis=a)
+            ahead = np.roll(f4, -1, axis=a)
+        else:
+            idx = np.arange(f4.shape[a])
+            back = np.take(f4, np.maximum(idx - 1, 0), axis=a)
+            ahead = np.take(f4, np.minimum(idx + 1, f4.shape[a] - 1), axis=a)
+        d = np.where(k > 0.0, f4 - back, ahead - f4)
+        acc = acc - k * d
+    return acc.reshape(nx * ny * nz, n)
