@@ -166,7 +166,9 @@ def test_korobov_stencil_bitexact(n, M, s0):
         assert np.all(np.diff(seg) >= 0)
 
 
-@pytest.mark.parametrize("cells", [1, 3])
+# 1 and 3 cells: node-kernel ranges follow the gather groups (256 and 88
+# blocks per cell); 40 cells: 7 blocks per cell, plain contiguous ranges
+@pytest.mark.parametrize("cells", [1, 3, 40])
 def test_korobov_collide_matches_oracle(cells):
     say("korobov collide", cells)
     vm = 4.8
