@@ -123,7 +123,8 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
             const int sl = slot[s]++;
             for (int k = 0; k < node_blocks[s]; k++) ni.push_back(make_int4(s, c, k, sl));
             for (int k = 0; k < KOR_NGROUPS * h->kntile; k++) gi.push_back(make_int4(s, c, k, sl));
-            for (int k = 0; k < (h->n_pnt + 255) / 256; k++) ai.push_back(make_int4(s, c, k, sl));
+            for (int k = 0; k < (h->n_pnt + KOR_APPLY_THREADS - 1) / KOR_APPLY_THREADS; k++)
+                ai.push_back(make_int4(s, c, k, sl));
         }
         sg.n_node = (int)ni.size();
         sg.n_gather = (int)gi.size();

@@ -523,7 +523,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // share of the moment sums) are issued together at the start, so the block
 // waits for memory once.
 #define APPLY_ROUNDS 2  // pairs of block sums per thread and moment in flight: up to 1,024 partials in one round
-#define APPLY_THREADS 256
+#define APPLY_THREADS KOR_APPLY_THREADS
 // Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
 // mp [20][nparts], fc [npts], Lp / Qp [G][npts], outputs [npts], X [5].
 template <bool CORR, int G>
@@ -849,10 +849,10 @@ int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const doub
                      int cells, double dt, double *f_upd)
 {
     GDT_CHECK(G == 1 || G == KOR_GROUPS, GDT_E_ARG, "kor_apply: %d partials per node", G);
-    const dim3 grid((h->n_pnt + 255) / 256, cells);
+    const dim3 grid((h->n_pnt + APPLY_THREADS - 1) / APPLY_THREADS, cells);
     auto kern = correct ? (G == 1 ? k_kor_apply<true, 1> : k_kor_apply<true, KOR_GROUPS>)
                         : (G == 1 ? k_kor_apply<false, 1> : k_kor_apply<false, KOR_GROUPS>);
-    GDT_LAUNCH_TIMED(h, "kor_apply", kern, grid, dim3(256), 0u, h->n_pnt, nparts, mpart, h->xi.as<double>(), f, Lpart,
+    GDT_LAUNCH_TIMED(h, "kor_apply", kern, grid, dim3(APPLY_THREADS), 0u, h->n_pnt, nparts, mpart, h->xi.as<double>(), f, Lpart,
                      Qpart, Qout, Lout, nu_out, X, dt, f_upd);
     GDT_HIP(hipGetLastError());
     return GDT_OK;
