@@ -37,7 +37,7 @@ void gdt_set_error(int code, const char *fmt, ...);
 #define KOR_TILES_MAX 64  // default gather tiles per group: G * 64 blocks, two per CU, one round
 #define KOR_TILES_LIMIT 1024
 #define KOR_NPTS_MAX 19200 // grid nodes the packed node records and the LDS copy of f admit
-#define KOR_APPLY_THREADS 256 // destinations per apply block
+#define KOR_APPLY_THREADS 128 // destinations per apply block (128: 26.0 us per step; 64: 27.0, 256: 26.3, 512: 27.9)
 
 // Device buffer owned by a context.
 struct DevBuf {

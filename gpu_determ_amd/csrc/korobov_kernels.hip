@@ -522,7 +522,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // All global loads of a thread (its node's partials, f and centre, and its
 // share of the moment sums) are issued together at the start, so the block
 // waits for memory once.
-#define APPLY_ROUNDS 2  // pairs of block sums per thread and moment in flight: up to 1,024 partials in one round
+#define APPLY_ROUNDS 2  // pairs of block sums per thread and moment in flight: 4 APPLY_THREADS partials a round
 #define APPLY_THREADS KOR_APPLY_THREADS
 // Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
 // mp [20][nparts], fc [npts], Lp / Qp [G][npts], outputs [npts], X [5].
