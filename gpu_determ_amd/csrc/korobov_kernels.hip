@@ -524,7 +524,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
 // waits for memory once.
 #define APPLY_ROUNDS 2  // pairs of block sums per thread and moment in flight: 4 APPLY_THREADS partials a round
 #define APPLY_THREADS KOR_APPLY_THREADS
-// Block b of a cell (destinations b*256 + tid).  Pointers are the cell's:
+// Block b of a cell (destinations b * APPLY_THREADS + tid).  Pointers are the cell's:
 // mp [20][nparts], fc [npts], Lp / Qp [G][npts], outputs [npts], X [5].
 template <bool CORR, int G>
 __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int npts, int nparts,
@@ -546,7 +546,7 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
     if (CORR) {
         // block sums of the moments: thread t adds blocks t, t + 256, ...
         // (coalesced, all loads in flight), then DPP row sums; wave 0 adds
-        // the 16 row sums in a fixed order (lane a: moment a) and solves
+        // the APPLY_THREADS / 16 row sums in a fixed order (lane a: moment a) and solves
         // (every block gets the same X)
         double m[20];
 #pragma unroll
