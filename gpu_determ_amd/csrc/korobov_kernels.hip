@@ -544,10 +544,10 @@ __device__ __forceinline__ void apply_block(double (*red)[20], double *Xs, int n
         lg[g] = __builtin_nontemporal_load(&Lp[(size_t)g * npts + vc]);
     }
     if (CORR) {
-        // block sums of the moments: thread t adds blocks t, t + 256, ...
-        // (coalesced, all loads in flight), then DPP row sums; wave 0 adds
-        // the APPLY_THREADS / 16 row sums in a fixed order (lane a: moment a) and solves
-        // (every block gets the same X)
+        // moment partials: thread t adds the partial pairs t, t + APPLY_THREADS,
+        // ... of every moment (coalesced, all loads in flight), then DPP row
+        // sums; wave 0 adds the APPLY_THREADS / 16 row sums in a fixed order
+        // (lane a: moment a) and solves (every block gets the same X)
         double m[20];
 #pragma unroll
         for (int a = 0; a < 20; a++) m[a] = 0.0;
