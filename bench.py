@@ -401,8 +401,8 @@ def main():
     G_ = tl["groups"]              # node groups of the gather (one per XCD)
     nparts = G_ * tl["tiles"]      # gather tiles per cell: one moment partial each
     alg = {
-        # node records (uint4 + D2) + f in; (D, 2wP) per node out
-        "kor_node": nn * 20 + 8 * n + nn * 16,
+        # node records (one uint4 each) + f in; (D, 2wP) per node out
+        "kor_node": nn * 16 + 8 * n + nn * 16,
         # entry records (node|role, num|den) + node values (each read at least
         # once) + group row pointers + centres in; group partials of Q and
         # loss + moment sums out

@@ -158,7 +158,6 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
             C.nn = h->nn;
             C.K = h->kK;
             C.nrec = h->kn4.as<uint4>();
-            C.nd2 = h->kn1.as<uint32_t>();
             C.tiles = h->ktile.as<int4>();
             C.gptr = h->kgptr.as<int32_t>();
             C.ent = h->kgent.as<uint2>();

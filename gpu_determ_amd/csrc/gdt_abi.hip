@@ -314,7 +314,7 @@ int gdt_destroy(gdt_handle h)
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     DevBuf *bufs[] = {&h->box2c, &h->cidx,   &h->xi,     &h->cmp2box, &h->tb,     &h->ta,    &h->gptr,  &h->gbox,
                       &h->gval,  &h->kidx,   &h->kr,     &h->kw,      &h->knode,  &h->kptr,  &h->kent,  &h->kn4,
-                      &h->kn1,   &h->ke4,    &h->ke1,    &h->kwb,     &h->kgptr,  &h->kgent, &h->ktile, &h->kmath,
+                      &h->ke4,    &h->ke1,    &h->kwb,     &h->kgptr,  &h->kgent, &h->ktile, &h->kmath,
                       &h->mx_bh, &h->mx_bl,  &h->mx_bf,  &h->mx_ah,   &h->mx_al,  &h->mx_af, &h->s_mx,  &h->s_fbox,
                       &h->s_out1, &h->s_out2, &h->s_vals, &h->s_loss, &h->s_Q,   &h->s_nu,  &h->s_tmp, &h->s_dt,
                       &h->s_mpart, &h->s_part};
@@ -578,22 +578,32 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
 
     h->nn = nn;  // read by the wave partition below
     h->nent = ne;
-    // packed records (16-bit node indices, the f of a cell in LDS)
+    // packed 16-byte node records (the f of a cell in LDS): 15-bit node
+    // indices, num / 4 and den / 4 in 11 bits each (the energies of a cubic
+    // grid's node pairs are all congruent mod 4, so both are multiples of 4,
+    // of one sign: r = |num| / |den| >= 0), D2 in 12 bits
     bool stream_ok = np <= KOR_NPTS_MAX;
-    for (long t = 0; t < nn && stream_ok; t++)
-        stream_ok = std::abs(cint[3 * t]) < 32768 && std::abs(cint[3 * t + 1]) < 32768 && cint[3 * t + 2] < 4096;
+    for (long t = 0; t < nn && stream_ok; t++) {
+        const int32_t num = cint[3 * t], den = cint[3 * t + 1], d2 = cint[3 * t + 2];
+        stream_ok = num % 4 == 0 && den % 4 == 0 && std::abs(num) / 4 <= 2047 && std::abs(den) / 4 <= 2047 &&
+                    ((num >= 0 && den >= 0) || (num <= 0 && den <= 0)) && d2 >= 0 && d2 < 4096;
+    }
     GDT_CHECK(stream_ok, GDT_E_ARG,
               "gdt_korobov_build: grid of %d nodes is beyond the packed-record limits (<= 19,200 nodes)", np);
     h->have_stream = false;
     h->wb_blocks = 0;
     h->h_kptr = ptr;
     {
+        // (a | b << 15 | D2 >> 10 << 30, l | m << 15, ls | ms << 15,
+        //  |num| / 4 | |den| / 4 << 11 | (D2 & 1023) << 22)
         auto pack = [&](long t, uint32_t *o4) {
             const int32_t *o = &cidx6[6 * (size_t)t];
-            o4[0] = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-            o4[1] = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-            o4[2] = (uint32_t)o[4] | ((uint32_t)o[5] << 16);
-            o4[3] = ((uint32_t)cint[3 * t] & 0xFFFFu) | (((uint32_t)cint[3 * t + 1] & 0xFFFFu) << 16);
+            const uint32_t d2 = (uint32_t)cint[3 * t + 2];
+            const uint32_t num4 = (uint32_t)(std::abs(cint[3 * t]) / 4), den4 = (uint32_t)(std::abs(cint[3 * t + 1]) / 4);
+            o4[0] = (uint32_t)o[0] | ((uint32_t)o[1] << 15) | ((d2 >> 10) << 30);
+            o4[1] = (uint32_t)o[2] | ((uint32_t)o[3] << 15);
+            o4[2] = (uint32_t)o[4] | ((uint32_t)o[5] << 15);
+            o4[3] = num4 | (den4 << 11) | ((d2 & 1023u) << 22);
         };
         // Node order of the packed records (node values and gather): sorted
         // by the lambda destination, then by the lambda+s destination,
@@ -608,11 +618,8 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
             const int32_t lx = cidx6[6 * (size_t)x + 2], ly = cidx6[6 * (size_t)y + 2];
             return lx != ly ? lx < ly : cidx6[6 * (size_t)x + 4] < cidx6[6 * (size_t)y + 4];
         });
-        std::vector<uint32_t> n4(4 * (size_t)nn), n1(nn);
-        for (long p = 0; p < nn; p++) {
-            pack(order[p], &n4[4 * p]);
-            n1[p] = (uint32_t)cint[3 * (size_t)order[p] + 2];
-        }
+        std::vector<uint32_t> n4(4 * (size_t)nn);
+        for (long p = 0; p < nn; p++) pack(order[p], &n4[4 * p]);
         std::vector<int32_t> dest_of(ne);
         for (int v = 0; v < np; v++)
             for (int e = ptr[v]; e < ptr[v + 1]; e++) dest_of[e] = v;
@@ -622,12 +629,10 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
             pack(t, &e4[4 * e]);
             e1[e] = (uint32_t)cint[3 * t + 2] | (role << 12) | ((uint32_t)dest_of[e] << 14);
         }
-        GDT_CHECK(h->kn4.ensure(sizeof(uint32_t) * 4 * nn) == 0 && h->kn1.ensure(sizeof(uint32_t) * nn) == 0 &&
-                      h->ke4.ensure(sizeof(uint32_t) * 4 * ne) == 0 &&
+        GDT_CHECK(h->kn4.ensure(sizeof(uint32_t) * 4 * nn) == 0 && h->ke4.ensure(sizeof(uint32_t) * 4 * ne) == 0 &&
                       h->ke1.ensure(sizeof(uint32_t) * ne) == 0,
                   GDT_E_NOMEM, "gdt_korobov_build: record allocation failed");
         GDT_HIP(hipMemcpy(h->kn4.p, n4.data(), sizeof(uint32_t) * 4 * nn, hipMemcpyHostToDevice));
-        GDT_HIP(hipMemcpy(h->kn1.p, n1.data(), sizeof(uint32_t) * nn, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke4.p, e4.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke1.p, e1.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
         // per node-group CSRs of the gather over the sorted node positions p:
@@ -654,7 +659,7 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 pg[v + 1] = (int32_t)run;
             }
         }
-        // entry records (p << 2 | role, num | den << 16 of the node)
+        // entry records (p << 2 | role, the last word of the node's record)
         std::vector<uint32_t> gent(2 * (size_t)ne);
         std::vector<int32_t> fill(gp);
         for (long p = 0; p < nn; p++) {

@@ -77,11 +77,11 @@ struct gdt_context {
     double ks0 = 0.0, kK = 0.0;
     long nn = 0, nent = 0;
     DevBuf kidx, kr, kw, knode, kptr, kent;
-    // per-node records (uint4 index pairs + num | den, uint32 D2) and, for the
-    // fused path, per-entry records
-    // (uint4 as the node's, uint32 D2 | role << 12 | destination << 14)
+    // per-node records (one uint4: six 15-bit node indices, num / 4, den / 4
+    // and D2) and, for the fused path, per-entry records (uint4 as the
+    // node's, uint32 D2 | role << 12 | destination << 14)
     bool have_stream = false;
-    DevBuf kn4, kn1, ke4, ke1, kwb;
+    DevBuf kn4, ke4, ke1, kwb;
     DevBuf kgptr, kgent;          // per node-group CSRs of the gather (uint2 entry records)
     DevBuf ktile;                 // gather tiles (d0, d1, e0, e1), [group][kntile]
     DevBuf kmath;                 // log / exp tables of the node terms (KMT_SIZE doubles)
@@ -199,7 +199,6 @@ struct KorClassDev {
     long nn;
     double K;
     const uint4 *nrec;
-    const uint32_t *nd2;
     const int4 *tiles;
     const int32_t *gptr;
     const uint2 *ent;

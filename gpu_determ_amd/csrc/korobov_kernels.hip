@@ -44,28 +44,32 @@
 #define KOR_GROUPS KOR_NGROUPS
 
 // D and 2 w P of one node from the LDS copy of f (log / exp tables at ts).
-// x = (a | b << 16, l | m << 16, ls | ms << 16, num | den << 16), D2 in the
-// low 12 bits of y.
-__device__ __forceinline__ void node_terms(const uint4 x, const uint32_t y, const double *fs, const double *ts,
-                                           double K, double &D, double &W2P, double &r)
+// x = (a | b << 15 | D2 >> 10 << 30, l | m << 15, ls | ms << 15,
+//      |num| / 4 | |den| / 4 << 11 | (D2 & 1023) << 22): r = num / den is the
+// quotient of the quarters (the same rational, so the same rounded double).
+__device__ __forceinline__ int kor_num(uint32_t w) { return (int)(w & 0x7FFu); }
+__device__ __forceinline__ int kor_den(uint32_t w) { return (int)((w >> 11) & 0x7FFu); }
+__device__ __forceinline__ void node_terms(const uint4 x, const double *fs, const double *ts, double K, double &D,
+                                           double &W2P, double &r)
 {
 #pragma clang fp contract(off)
-    const int num = (int)(short)(x.w & 0xFFFFu), den = (int)(short)(x.w >> 16);
-    const double P = fs[x.x & 0xFFFFu] * fs[x.x >> 16];
-    const double Pl = fs[x.y & 0xFFFFu] * fs[x.y >> 16];
+    const int num = kor_num(x.w), den = kor_den(x.w);
+    const uint32_t d2 = (x.w >> 22) | ((x.x >> 30) << 10);
+    const double P = fs[x.x & 0x7FFFu] * fs[(x.x >> 15) & 0x7FFFu];
+    const double Pl = fs[x.y & 0x7FFFu] * fs[x.y >> 15];
     double R;
     r = 0.0;
     if (num == 0) {
         R = Pl;
     } else {
         r = (double)num / (double)den;
-        const double Ps = fs[x.z & 0xFFFFu] * fs[x.z >> 16];
+        const double Ps = fs[x.z & 0x7FFFu] * fs[x.z >> 15];
         if (Pl > 0.0 && Ps > 0.0)
             R = Pl * kmt_exp(r * kmt_log(Ps / Pl, ts), ts);
         else
             R = (1.0 - r) * Pl + r * Ps;
     }
-    const double w = K * sqrt((double)(y & 0xFFFu));
+    const double w = K * sqrt((double)d2);
     D = w * (P - R);
     W2P = 2.0 * (w * P);
 }
@@ -141,10 +145,9 @@ __device__ __forceinline__ uint4 load_nt_u4(const uint4 *p)
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-__device__ __forceinline__ void node_block(double *fs, int npts, long nn, const uint4 *__restrict__ nrec,
-                                           const uint32_t *__restrict__ nd2, double K, const double *__restrict__ fc,
-                                           double2 *__restrict__ out, int b, int B, const double *__restrict__ kmt,
-                                           bool xcd)
+__device__ __forceinline__ void node_block(double *fs, int npts, long nn, const uint4 *__restrict__ nrec, double K,
+                                           const double *__restrict__ fc, double2 *__restrict__ out, int b, int B,
+                                           const double *__restrict__ kmt, bool xcd)
 {
 #pragma clang fp contract(off)
     long lo = 0, hi = nn, k = b, S = B;
@@ -159,23 +162,16 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
     const long tend = t0 + per < hi ? t0 + per : hi;
     const long ta = t0 + threadIdx.x, tb = ta + KOR_THREADS;
     uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
-    uint32_t ya = 0u, yb = 0u;
     // streaming (nontemporal) loads: records are read once per step, and
     // should not push the gather's entries out of L2
-    if (ta < tend) {
-        xa = load_nt_u4(&nrec[ta]);
-        ya = __builtin_nontemporal_load(&nd2[ta]);
-    }
-    if (tb < tend) {
-        xb = load_nt_u4(&nrec[tb]);
-        yb = __builtin_nontemporal_load(&nd2[tb]);
-    }
+    if (ta < tend) xa = load_nt_u4(&nrec[ta]);
+    if (tb < tend) xb = load_nt_u4(&nrec[tb]);
     stage_f(fs, fc, npts, kmt);
     __syncthreads();
     const double *ts = fs + kor_fpad(npts);
     double Da, Wa, ra, Db, Wb, rb;
-    node_terms(xa, ya, fs, ts, K, Da, Wa, ra);
-    node_terms(xb, yb, fs, ts, K, Db, Wb, rb);
+    node_terms(xa, fs, ts, K, Da, Wa, ra);
+    node_terms(xb, fs, ts, K, Db, Wb, rb);
     // streaming (nontemporal) stores: the values are read by the next kernel
     // only, from other XCDs, so they need not linger in this XCD's L2
     if (ta < tend) {
@@ -188,20 +184,19 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
     }
     for (long t = tb + KOR_THREADS; t < tend; t += KOR_THREADS) {
         double D, W2P, r;
-        node_terms(nrec[t], nd2[t], fs, ts, K, D, W2P, r);
+        node_terms(nrec[t], fs, ts, K, D, W2P, r);
         __builtin_nontemporal_store(D, &out[t].x);
         __builtin_nontemporal_store(W2P, &out[t].y);
     }
 }
 
 __global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, const uint4 *__restrict__ nrec,
-                                                           const uint32_t *__restrict__ nd2, double K,
-                                                           const double *__restrict__ f, double2 *__restrict__ V,
-                                                           const double *__restrict__ kmt)
+                                                           double K, const double *__restrict__ f,
+                                                           double2 *__restrict__ V, const double *__restrict__ kmt)
 {
     extern __shared__ double fs[];
     const int c = blockIdx.y;
-    node_block(fs, npts, nn, nrec, nd2, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x, kmt,
+    node_block(fs, npts, nn, nrec, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x, kmt,
                gridDim.x % KOR_GROUPS == 0);
 }
 
@@ -245,7 +240,8 @@ __device__ __forceinline__ void zero_empty(int ga, int gb, const int32_t *ptr, d
 // ---------------------------------------------------------------------------
 // destination-sorted gather in tiles, one node group per XCD
 // ---------------------------------------------------------------------------
-// Entry record: (node << 2 | role, num | den << 16 of the node).  An entry's
+// Entry record: (node << 2 | role, the node record's last word: |num| / 4,
+// |den| / 4, D2 bits).  An entry's
 // term is, from the node's (D, 2 w P) and r = num / den (the node kernel's
 // operations, so the terms are bit-identical to the oracle's):
 //   role 0 (a, b):   q = -D,  l = 2 w P;   role 1 (l, m): q = (1 - r) D;
@@ -327,7 +323,7 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
                 const int e = j0 + j * TILE_THREADS;
                 if (e < w1) {
                     const uint32_t role = x[j].x & 3u;
-                    const int num = (int)(short)(x[j].y & 0xFFFFu), den = (int)(short)(x[j].y >> 16);
+                    const int num = kor_num(x[j].y), den = kor_den(x[j].y);
                     const double r = num == 0 ? 0.0 : (double)num / (double)den;
                     ql[e - w0] = role == 0u ? make_double2(-1.0 * v[j].x, v[j].y)
                                             : make_double2((role == 1u ? (1.0 - r) : r) * v[j].x, 0.0);
@@ -414,7 +410,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, 
 // fused: per-entry recomputation from the LDS copy of f
 // ---------------------------------------------------------------------------
 // One 20-byte record per entry of the global destination-sorted CSR: the
-// node's six indices and num | den (as the node record) and D2 | role << 12 |
+// node's record (six indices, num / 4, den / 4, D2) and D2 | role << 12 |
 // destination << 14.  Wave w of the grid owns the destinations
 // [wbound[w], wbound[w+1]) (whole destinations, balanced entry counts) and
 // walks their entries in chunks of 64: every lane evaluates its entry's term
@@ -476,7 +472,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_kor_fused(int npts, const int32
         int key = -2 - lane;
         if (valid) {
             double D, W2P, r;
-            node_terms(x, y, fs, ts, K, D, W2P, r);
+            node_terms(x, fs, ts, K, D, W2P, r);
             const uint32_t role = (y >> 12) & 3u;
             if (role == 0u) {
                 q = -1.0 * D;
@@ -657,7 +653,7 @@ __global__ __launch_bounds__(KOR_THREADS) void k_batch_node(const KorClassDev *_
         if (i != (int)blockIdx.x) __syncthreads();  // the previous item's LDS reads are done
         const int4 it = items[i];
         const KorClassDev C = cls[it.x];
-        node_block(fs, C.npts, C.nn, C.nrec, C.nd2, C.K, C.f + (size_t)it.y * C.npts, C.V + (size_t)it.w * C.nn,
+        node_block(fs, C.npts, C.nn, C.nrec, C.K, C.f + (size_t)it.y * C.npts, C.V + (size_t)it.w * C.nn,
                    it.z, C.node_blocks, C.kmt, false);
     }
 }
@@ -808,7 +804,7 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
 {
     const int B = kor_blocks(h, cells);
     GDT_LAUNCH_TIMED(h, "kor_node", k_kor_node, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->n_pnt, h->nn,
-                     h->kn4.as<uint4>(), h->kn1.as<uint32_t>(), h->kK, f, (double2 *)DW, h->kmath.as<double>());
+                     h->kn4.as<uint4>(), h->kK, f, (double2 *)DW, h->kmath.as<double>());
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
