@@ -41,6 +41,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 #define MX_KB 32          // bf16 K slice
 #define MX_KF 16          // fp32 K slice
 #define MX_PB 40          // bf16 LDS row pitch (elements)
+static_assert(MX_T == MX_KB * 16, "a bf16 T slice gives each thread one (n1, iz) pair");
 #define MX_PF 17          // fp32 LDS row pitch (elements)
 #define MX_GP 33          // LDS row pitch of a staged G chunk (floats)
 
@@ -152,12 +153,14 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
                     }
                 }
                 // T slice: 128 s x 32 n1, 8 per thread.  Thread q holds n1 =
-                // k0 + q % 32 and the column iz = q / 32 of the tile's eight iy
+                // k0 + q / 16 and the column iz = q % 16 of the tile's eight iy
                 // rows: n1, sx and sz (the x and z halves of A = (s + n) / 2
                 // and B = (s - n) / 2) are fixed; row by row sy grows by 2, so
-                // Ay and By step by one.
+                // Ay and By step by one.  Consecutive lanes take consecutive iz,
+                // so their reads of f are consecutive in z (0.8% faster than n1
+                // fastest, the same bits).
                 {
-                    const int nk = threadIdx.x % KB, iz = threadIdx.x / KB;
+                    const int nk = threadIdx.x / MX_N, iz = threadIdx.x % MX_N;
                     const MxTxz tx = mx_txz(p, sx_t, iz, k0 + nk);
                     const int iy0 = 8 * (t & 1);
 #pragma unroll 1
