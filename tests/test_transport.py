@@ -83,6 +83,12 @@ def test_gpu_transport_errors():
         ctx.transport(f, (3, 2, 1), h, dt_max * 1.01)  # Courant number above 1
     with pytest.raises(G.GdtError):
         ctx.transport(f, (3, 2, 1), h, 0.1, out=f)  # aliasing
+
+    class Shifted:  # f_out one cell into f: a partial overlap
+        ptr = f.ptr + 8 * ctx.n_pnt
+
+    with pytest.raises(G.GdtError):
+        ctx.transport(f, (3, 2, 1), h, 0.1, out=Shifted)
     with pytest.raises(G.GdtError):
         ctx.transport(f, (3, 2, 1), h, 0.1, (0, 2, 0))  # unknown boundary
 
