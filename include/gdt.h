@@ -125,10 +125,10 @@ int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32
 int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
 /* Limit of gather tiles per node group (default 64, 1..1024); no reference
  * counterpart (a launch-shape setting of the gather).  A tile holds at most
- * 512 destinations, so a group has at least ceil(nodes / 512) tiles whatever
- * the limit (34 at 32^3).  Fewer, larger tiles
- * give the apply fewer moment partials and the gather less per-tile work;
- * more tiles put more blocks in flight.  On a built stencil the tiles are
+ * 512 destinations, so a group has at least ceil(grid nodes / 512) tiles
+ * whatever the limit (34 at 32^3).  Fewer, larger tiles give the apply fewer
+ * moment partials and the gather less per-tile work; more tiles put more
+ * blocks in flight.  On a built stencil the tiles are
  * rebuilt at once (after the context's stream drains).  Per-destination sums
  * do not depend on the tiling; the moment sums do, in their last bits.  A
  * batch holding this context must be configured again (gdt_batch_config);
