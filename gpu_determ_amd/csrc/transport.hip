@@ -24,21 +24,32 @@
 
 #define TR_THREADS 256
 
-// Block (x, y): velocities x * TR_THREADS + tid, lines of cells (ix, iy) =
-// y, y + gridDim.y, ... walked along z.  A thread keeps its velocity's k_a in
-// registers and the z neighbours in a sliding window, so per cell it reads f
-// of the cell ahead in z and of its upwind x and y neighbours.
-__global__ __launch_bounds__(TR_THREADS) void k_transport(int n, int nx, int ny, int nz, double cx, double cy,
+// A block takes one work unit (a velocity block of TR_THREADS velocities, a
+// line of cells (ix, iy)) and walks the line along z.  A thread keeps its
+// velocity's k_a in registers and the z neighbours in a sliding window, so
+// per cell it reads f of the cell ahead in z and of its upwind x and y
+// neighbours.
+__global__ __launch_bounds__(TR_THREADS) void k_transport(int nvb, int n, int nx, int ny, int nz, double cx, double cy,
                                                           double cz, int bcx, int bcy, int bcz,
                                                           const double *__restrict__ xi,
                                                           const double *__restrict__ f, double *__restrict__ out)
 {
 #pragma clang fp contract(off)
-    const int v = blockIdx.x * TR_THREADS + threadIdx.x;
+    // work unit u = (velocity block, ix, iy), iy fastest then ix then the
+    // velocity block; hardware block b runs on XCD b % 8, so it takes unit
+    // (b % 8) * per + b / 8: each XCD walks one contiguous eighth of the unit
+    // order, and a line's x and y neighbours were read recently on the same
+    // XCD (3.36 -> 3.48 TB/s on 16^3 cells of the 24^3 grid, with the z loop
+    // unrolled by 4)
+    const long units = (long)nvb * nx * ny, per = (units + 7) / 8;
+    const long u = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (u >= units) return;
+    const int vb = (int)(u / ((long)nx * ny)), line = (int)(u % ((long)nx * ny));
+    const int v = vb * TR_THREADS + threadIdx.x;
     if (v >= n) return;  // no barrier below: threads past the last velocity just leave
     const double kx = cx * xi[3 * v], ky = cy * xi[3 * v + 1], kz = cz * xi[3 * v + 2];
     const size_t N = (size_t)n;
-    for (int line = blockIdx.y; line < nx * ny; line += gridDim.y) {
+    {
         const int ix = line / ny, iy = line % ny;
         // upwind neighbour lines in x and y (the line itself where a
         // zero-gradient boundary has none)
@@ -48,6 +59,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_transport(int n, int nx, int ny,
         const size_t base = (size_t)line * nz, bx = ((size_t)px * ny + iy) * nz, by = ((size_t)ix * ny + py) * nz;
         double cur = f[base * N + v];
         double prev = bcz == GDT_BC_PERIODIC ? f[(base + nz - 1) * N + v] : cur;
+#pragma unroll 4
         for (int iz = 0; iz < nz; iz++) {
             const double next = iz + 1 < nz ? f[(base + iz + 1) * N + v]
                                             : (bcz == GDT_BC_PERIODIC ? f[base * N + v] : cur);
@@ -98,8 +110,9 @@ int gdt_transport(gdt_handle h, const double *f, double *f_out, int nx, int ny, 
     }
     GDT_HIP(hipSetDevice(h->device));
     const int vblocks = (n + TR_THREADS - 1) / TR_THREADS;
-    const int lines = (int)std::min<long>((long)nx * ny, 65535);
-    hipLaunchKernelGGL(k_transport, dim3(vblocks, lines), dim3(TR_THREADS), 0, h->stream, n, nx, ny, nz, c[0], c[1],
+    const long units = (long)vblocks * nx * ny, grid = (units + 7) / 8 * 8;
+    GDT_CHECK(grid <= 0x7FFFFFFFL, GDT_E_ARG, "gdt_transport: %ld work units", units);
+    hipLaunchKernelGGL(k_transport, dim3((unsigned)grid), dim3(TR_THREADS), 0, h->stream, vblocks, n, nx, ny, nz, c[0], c[1],
                        c[2], bc[0], bc[1], bc[2], h->xi.as<double>(), f, f_out);
     GDT_HIP(hipGetLastError());
     return GDT_OK;
