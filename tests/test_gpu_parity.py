@@ -201,17 +201,18 @@ class _DeviceView:
         self.ptr = base.ptr + 8 * offset_doubles
 
 
-@pytest.mark.parametrize("n,offset", [((15, 15, 15), 0), ((20, 20, 20), 1), ((20, 20, 20), 0)])
-def test_korobov_collide_staging_paths(n, offset):
+@pytest.mark.parametrize("n,offset,R", [((15, 15, 15), 0, 4.8), ((20, 20, 20), 1, 4.8), ((20, 20, 20), 0, 4.8),
+                                        ((16, 16, 16), 0, -1.0)])
+def test_korobov_collide_staging_paths(n, offset, R):
     """The node kernel stages a cell's f into LDS by 16-byte DMA when f is
     16-byte aligned and the grid's node count is even, else by 8-byte loads.
     An odd grid (1,791 nodes: odd counts, every second cell 8-byte aligned
     only) and an even grid whose f starts 8 bytes into its buffer take the
     second path; all match the oracle, and the even grid's two paths give the
-    same bits."""
-    say("korobov staging paths", n, offset)
+    same bits.  The 16^3 box (no sphere cut, 4,096 nodes) takes the first."""
+    say("korobov staging paths", n, offset, R)
     vm = 4.8
-    g = O.Grid(n, (-vm,) * 3, (vm,) * 3, vm)
+    g = O.Grid(n, (-vm,) * 3, (vm,) * 3, R)
     c = ctx_for(g)
     M = 1 << 16
     c.korobov_build(M, seed=5, s0=0.25)
