@@ -382,3 +382,28 @@ def test_kmath_pow_accuracy():
     assert ratio.max() <= 1.0, (x[ratio.argmax()], r[ratio.argmax()], err.max())
     # special values to within two ulps: 1^0.5, 0.5^1, 2^1, 1.5^0
     assert np.all(np.abs(got[[0, 1, 2, 8]] - [1.0, 0.5, 2.0, 1.0]) <= 2 * np.spacing([1.0, 0.5, 2.0, 1.0]))
+
+
+def test_korobov_gather_multipass():
+    """Tiles with more entries than one LDS pass holds (16 tiles per group at
+    20^3 and 2^20 nodes: about 8,700 entries per tile against a pass of at most
+    4,864) are summed in passes; the result still matches the oracle, and the
+    default tiling to FP64 rounding."""
+    say("korobov gather multipass")
+    vm = 4.8
+    g = O.Grid((20, 20, 20), (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    M = 1 << 20
+    c.korobov_build(M, seed=9, s0=0.25)
+    st = O.korobov_stencil(g, M, c.korobov_info()["a"], 0.25)
+    rng = np.random.default_rng(13)
+    f = np.stack([mixture(g.xi, rng) for _ in range(2)])
+    fd = G.DeviceArray.from_host(f)
+    l1, q1 = c.collide_korobov(fd, 2)
+    c.korobov_set_tiles(16)
+    tl = c.korobov_tiles()
+    assert tl["tiles"] <= 16 and c.korobov_info()["entries"] / (tl["groups"] * tl["tiles"]) > tl["cap"]
+    l2, q2 = c.collide_korobov(fd, 2)
+    ol, oq = O.korobov_collide(g, st, f)
+    assert rel_l2(q2.get(), oq) < 1e-12 and rel_l2(l2.get(), ol) < 1e-12
+    assert rel_l2(q2.get(), q1.get()) < 1e-14
