@@ -54,9 +54,13 @@ def test_oracle_conservation_and_max_principle():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("bc", [(0, 0, 0), (1, 1, 1), (0, 1, 0)])
-def test_gpu_transport_matches_oracle(bc):
-    print("transport", bc, flush=True)
-    ctx = G.Context((12, 12, 12), (-4.8,) * 3, (4.8,) * 3, 4.8)
+@pytest.mark.parametrize("gn", [12, 11])
+def test_gpu_transport_matches_oracle(bc, gn):
+    """12^3 has 912 velocity nodes (even: two velocities per thread, 16-B
+    accesses), 11^3 has 739 (odd: one velocity per thread)."""
+    print("transport", gn, bc, flush=True)
+    ctx = G.Context((gn,) * 3, (-4.8,) * 3, (4.8,) * 3, 4.8)
+    assert ctx.n_pnt % 2 == (0 if gn == 12 else 1)
     shape, h = (5, 4, 3), (0.5, 0.4, 0.6)
     rng = np.random.default_rng(3)
     f = np.stack([mixture(ctx.xi, rng) for _ in range(60)])
