@@ -337,7 +337,16 @@ def cpu_baseline(xi, f0, n_pnt, d3v, budget_s=15.0):
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / med, "unit": "Q(f,f) evals/s", "cores": threads, "kind": "port",
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": 1.0 / med, "unit": "Q(f,f) evals/s", "cores": threads, "kind": "port", "cpu_model": model,
             "sample": "%d corrected evaluations + Euler steps of the same 32^3 / 2^21-node cell "
                       "(median of %d, stencil build untimed)" % (len(times), len(times))}
 
