@@ -170,8 +170,6 @@ __global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int 
     const int bi = blk / (bv * bw), bj = (blk / bw) % bv, bk = blk % bw;
     const int i = bi * 4 + lane / 16, j = bj * 4 + (lane / 4) % 4, k = bk * 4 + lane % 4;
     const bool act = i < nu && j < nv && k < nw;
-    const int hv = nv / 2, hw = nw / 2;
-
     double acc = 0.0;
     for (int gi = 0; gi < nu; gi++)
         for (int gj = 0; gj < nv; gj++)
@@ -179,7 +177,6 @@ __global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int 
                 const int g = (gi * nv + gj) * nw + gk;
                 const int e0 = gptr[g], e1 = gptr[g + 1];
                 if (e0 == e1) continue;
-                const int pi = gi & 1, pj = gj & 1, pk = gk & 1;
                 for (int sgn = 0; sgn < 8; sgn++) {
                     if (((sgn & 1) && gi == 0) || ((sgn & 2) && gj == 0) || ((sgn & 4) && gk == 0)) continue;
                     const int i1 = (sgn & 1) ? i + gi : i - gi;
@@ -191,8 +188,7 @@ __global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int 
                     for (int e = e0; e < e1; e++) {
                         const int box = gbox[e];
                         const double val = gval[e];
-                        const int kn1 = box % hw, jn1 = (box / hw) % hv, in1 = box / (hw * hv);
-                        const int in = (in1 << 1) | pi, jn = (jn1 << 1) | pj, kn = (kn1 << 1) | pk;
+                        const int in = box & 1023, jn = (box >> 10) & 1023, kn = box >> 20;
                         const int ih = (SI + in) >> 1, im = SI - in;
                         const int jh = (SJ + jn) >> 1, jm = SJ - jn;
                         const int kh = (SK + kn) >> 1, km = SK - kn;

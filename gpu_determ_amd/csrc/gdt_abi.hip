@@ -369,6 +369,9 @@ int gdt_build_tables(gdt_handle h, int mode)
     for (int a = 0; a < 3; a++)
         GDT_CHECK(h->n[a] % 2 == 0, GDT_E_ARG, "gdt_build_tables: n[%d] = %d must be even", a, h->n[a]);
     GDT_CHECK(h->n_box <= 32768, GDT_E_ARG, "gdt_build_tables: %d box nodes (max 32768)", h->n_box);
+    for (int a = 0; a < 3; a++)
+        GDT_CHECK(h->n[a] <= GLIST_AXIS_MAX, GDT_E_ARG, "gdt_build_tables: n[%d] = %d (max %d)", a, h->n[a],
+                  GLIST_AXIS_MAX);
     const int rc = launch_build_tables(h, mode);
     if (rc != GDT_OK) return rc;
     h->table_mode = mode;
@@ -393,7 +396,8 @@ int gdt_set_tables(gdt_handle h, const double *b, const double *a)
     H_CHECK(h);
     GDT_CHECK(b && a, GDT_E_ARG, "gdt_set_tables: null table");
     for (int ax = 0; ax < 3; ax++)
-        GDT_CHECK(h->n[ax] % 2 == 0, GDT_E_ARG, "gdt_set_tables: n[%d] must be even", ax);
+        GDT_CHECK(h->n[ax] % 2 == 0 && h->n[ax] <= GLIST_AXIS_MAX, GDT_E_ARG,
+                  "gdt_set_tables: n[%d] = %d must be even and at most %d", ax, h->n[ax], GLIST_AXIS_MAX);
     const int nb = h->n_box;
     const long n_a = (long)nb * nb / 8;
     GDT_CHECK(h->tb.ensure(sizeof(double) * nb) == 0 && h->ta.ensure(sizeof(double) * n_a) == 0, GDT_E_NOMEM,
@@ -416,7 +420,7 @@ int gdt_set_tables(gdt_handle h, const double *b, const double *a)
                     const double v =
                         ha[((long)((in1 * nu + gi) * (hv * hw) + jn1 * hw + kn1)) * nyz + gj * nw + gk];
                     if (v != 0.0) {
-                        gb.push_back((in1 * hv + jn1) * hw + kn1);
+                        gb.push_back(glist_pack(in1 << 1 | (gi & 1), jn1 << 1 | (gj & 1), kn1 << 1 | (gk & 1)));
                         gv.push_back(v);
                     }
                 }

@@ -236,6 +236,13 @@ __host__ __device__ constexpr long kor_group_begin(int g, long nn)
 // LDS copy of f padded to whole 128-double chunks (the node kernels' 16-byte
 // staging writes whole wave chunks); the log / exp tables follow the padding
 __host__ __device__ constexpr int kor_fpad(int npts) { return (npts + 127) & ~127; }
+
+// Gain-list entry of relative-velocity triple (gi, gj, gk): the box offsets
+// in = 2 in1 + (gi & 1), jn, kn (each < its axis size), 10 bits each, so the
+// gain kernel splits an entry with shifts and masks.  Axes of the matrix
+// tables are at most GLIST_AXIS_MAX.
+#define GLIST_AXIS_MAX 1024
+__host__ __device__ constexpr int glist_pack(int in, int jn, int kn) { return in | jn << 10 | kn << 20; }
 int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells);
 // G group partials [cell][G][node] of Q and loss (G = 1: plain arrays)
 int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const double *mpart, const double *f,

@@ -157,7 +157,7 @@ __global__ void k_table_a(TableParams P, double *__restrict__ a, long n_a)
 }
 
 // Per relative-velocity triple: count / list the nonzero boxes in
-// (in1, jn1, kn1) order.  Box packed as (in1*hv + jn1)*hw + kn1.
+// (in1, jn1, kn1) order, each as glist_pack(in, jn, kn).
 __global__ void k_glist(int nu, int nv, int nw, const double *__restrict__ a, const int *__restrict__ gptr,
                         int *__restrict__ gcount, int *__restrict__ gbox, double *__restrict__ gval)
 {
@@ -175,7 +175,7 @@ __global__ void k_glist(int nu, int nv, int nw, const double *__restrict__ a, co
                 const double v = a[ai];
                 if (v != 0.0) {
                     if (gptr) {
-                        gbox[base + cnt] = (in1 * hv + jn1) * hw + kn1;
+                        gbox[base + cnt] = glist_pack(in1 << 1 | (gi & 1), jn1 << 1 | (gj & 1), kn1 << 1 | (gk & 1));
                         gval[base + cnt] = v;
                     }
                     cnt++;

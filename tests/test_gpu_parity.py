@@ -68,6 +68,32 @@ def test_loss_gain_8cube():
     assert rel_l2(gain, g.to_cmp(og)) < 1e-13
 
 
+def test_set_tables_lists_match_built():
+    """gdt_set_tables derives the nonzero-box lists on the host; with the
+    GPU-built tables of another handle, its gain equals that handle's bit for
+    bit, and with the oracle's tables it matches the oracle's gain."""
+    say("set_tables 8^3 and 10^3")
+    rng = np.random.default_rng(5)
+    for n in (8, 10):
+        g = O.Grid((n,) * 3, (-4, -4, -4), (4, 4, 4), -1.0)
+        a_ctx, b_ctx = ctx_for(g), ctx_for(g)
+        a_ctx.build_tables(0)
+        hb, ha, nnz = a_ctx.tables()
+        db, da = G.DeviceArray.from_host(hb), G.DeviceArray.from_host(ha)  # held across the call
+        G.check(G.lib().gdt_set_tables(b_ctx.h, db.ptr, da.ptr))
+        assert b_ctx.tables()[2] == nnz
+        f = np.stack([mixture(g.xi, rng) for _ in range(2)])
+        fd = G.DeviceArray.from_host(f)
+        ga = a_ctx.collide_matrix(fd, 2)[1].get()
+        gb = b_ctx.collide_matrix(fd, 2)[1].get()
+        assert np.array_equal(ga, gb)
+        ob, oa = O.build_tables(g, 0)
+        db, da = G.DeviceArray.from_host(ob), G.DeviceArray.from_host(oa)
+        G.check(G.lib().gdt_set_tables(b_ctx.h, db.ptr, da.ptr))
+        og = g.to_cmp(O.gain(g, oa, g.to_box(f)))
+        assert rel_l2(b_ctx.collide_matrix(fd, 2)[1].get(), og) < 1e-13
+
+
 def test_matrix_sphere_cut_16():
     say("matrix sphere-cut 16^3")
     g = O.Grid((16, 16, 16), (-4.8, -4.8, -4.8), (4.8, 4.8, 4.8), 4.8)
