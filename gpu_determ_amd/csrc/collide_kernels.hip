@@ -152,15 +152,21 @@ int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells)
 // g (wave-uniform), the 8 partner signs v1 = v +- g per axis, and the g's
 // nonzero boxes (scalar loads).  Sign choices with no partner inside the grid
 // for any lane of the block are skipped as a whole.
+// Rows of f in LDS are padded to gain_row(nw) = nw rounded up to 4 mod 8
+// doubles: the 16 lanes of a wave that share i read 4 rows j of 4 k each, and
+// with a row stride of 4 mod 8 doubles (8 or 24 banks of 4 bytes mod 32) the
+// four rows fall on disjoint banks.
+__host__ __device__ constexpr int gain_row(int nw) { return nw + ((12 - nw % 8) % 8); }
+
 __global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int *__restrict__ gptr,
                                               const int *__restrict__ gbox, const double *__restrict__ gval,
                                               const double *__restrict__ f, double *__restrict__ out)
 {
     extern __shared__ double fs[];
-    const int nyz = nv * nw, nb = nu * nyz;
+    const int nb = nu * nv * nw, rw = gain_row(nw), nyz = nv * rw;
     const int c = blockIdx.y;
     const double *fc = f + (size_t)c * nb;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) fs[t] = fc[t];
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) fs[(t / nw) * rw + t % nw] = fc[t];
     __syncthreads();
 
     const int bu = (nu + 3) / 4, bv = (nv + 3) / 4, bw = (nw + 3) / 4;
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int 
                         const bool valid = ok && ih < nu && im >= 0 && jh < nv && jm >= 0 && kh < nw && km >= 0;
                         if (valid) {
                             const int il = im >> 1, jl = jm >> 1, kl = km >> 1;
-                            const int Ih = ih * nyz, Il = il * nyz, Jh = jh * nw, Jl = jl * nw;
+                            const int Ih = ih * nyz, Il = il * nyz, Jh = jh * rw, Jl = jl * rw;
                             const double t4 = fs[Ih + Jh + kh] * fs[Il + Jl + kl] + fs[Ih + Jl + kl] * fs[Il + Jh + kh] +
                                               fs[Ih + Jh + kl] * fs[Il + Jl + kh] + fs[Ih + Jl + kh] * fs[Il + Jh + kl];
                             acc += val * t4;
@@ -210,7 +216,7 @@ int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells
 {
     const int nu = h->n[0], nv = h->n[1], nw = h->n[2];
     const int blocks = ((nu + 3) / 4) * ((nv + 3) / 4) * ((nw + 3) / 4);
-    const size_t lds = sizeof(double) * h->n_box;
+    const size_t lds = sizeof(double) * nu * nv * gain_row(nw);
     GDT_CHECK(lds <= 160 * 1024, GDT_E_ARG, "gain: grid of %d box nodes exceeds the LDS budget", h->n_box);
     KernelTimer kt(h, "gain");
     hipLaunchKernelGGL(k_gain, dim3((blocks + 3) / 4, cells), dim3(256), lds, h->stream, nu, nv, nw,
