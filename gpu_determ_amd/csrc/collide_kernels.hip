@@ -151,14 +151,17 @@ int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells)
 // owns a 4x4x4 block of destinations; it walks the relative-velocity triples
 // g (wave-uniform), the 8 partner signs v1 = v +- g per axis, and the g's
 // nonzero boxes (scalar loads).  Sign choices with no partner inside the grid
-// for any lane of the block are skipped as a whole.
+// for any lane of the block are skipped as a whole.  Blocks of 8 waves share
+// one LDS copy of f: f takes 32-64 KB at 16^3-20^3, so 4-wave blocks left 2-4
+// waves per SIMD to hide each entry's scalar loads (8 waves: 414 -> 509
+// evaluations per second at 16^3, 49 -> 70 at 20^3; 16 waves: 496, 70).
 // Rows of f in LDS are padded to gain_row(nw) = nw rounded up to 4 mod 8
 // doubles: the 16 lanes of a wave that share i read 4 rows j of 4 k each, and
 // with a row stride of 4 mod 8 doubles (8 or 24 banks of 4 bytes mod 32) the
 // four rows fall on disjoint banks.
 __host__ __device__ constexpr int gain_row(int nw) { return nw + ((12 - nw % 8) % 8); }
 
-__global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int *__restrict__ gptr,
+__global__ __launch_bounds__(512) void k_gain(int nu, int nv, int nw, const int *__restrict__ gptr,
                                               const int *__restrict__ gbox, const double *__restrict__ gval,
                                               const double *__restrict__ f, double *__restrict__ out)
 {
@@ -171,7 +174,7 @@ __global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int 
 
     const int bu = (nu + 3) / 4, bv = (nv + 3) / 4, bw = (nw + 3) / 4;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const int blk = blockIdx.x * 4 + wave;
+    const int blk = blockIdx.x * 8 + wave;
     if (blk >= bu * bv * bw) return;  // whole wave exits; no barrier follows
     const int bi = blk / (bv * bw), bj = (blk / bw) % bv, bk = blk % bw;
     const int i = bi * 4 + lane / 16, j = bj * 4 + (lane / 4) % 4, k = bk * 4 + lane % 4;
@@ -190,20 +193,31 @@ __global__ __launch_bounds__(256) void k_gain(int nu, int nv, int nw, const int 
                     const int k1 = (sgn & 4) ? k + gk : k - gk;
                     const bool ok = act && i1 >= 0 && i1 < nu && j1 >= 0 && j1 < nv && k1 >= 0 && k1 < nw;
                     if (!__any(ok)) continue;
+                    // The pair of a box offset (in, jn, kn) is ih = (SI + in) / 2,
+                    // il = (SI - in) / 2 per axis: SI = 2 i +- gi and in carry
+                    // the parity of gi, so both halves are exact.  Hence
+                    //   ih < nu and il >= 0  <=>  in <= min(SI, 2 nu - 1 - SI)
+                    // (one bound per lane and axis; -1 folds in a partner off
+                    // the grid), and the LDS index of (ih or il, jh or jl,
+                    // kh or kl) is (P +- in nyz +- jn rw +- kn) / 2 with
+                    // P = SI nyz + SJ rw + SK: a lane value plus a
+                    // wave-uniform offset.
                     const int SI = i + i1, SJ = j + j1, SK = k + k1;
+                    const int LI = ok ? min(SI, 2 * nu - 1 - SI) : -1;
+                    const int LJ = min(SJ, 2 * nv - 1 - SJ), LK = min(SK, 2 * nw - 1 - SK);
+                    const int P = SI * nyz + SJ * rw + SK;
                     for (int e = e0; e < e1; e++) {
                         const int box = gbox[e];
                         const double val = gval[e];
                         const int in = box & 1023, jn = (box >> 10) & 1023, kn = box >> 20;
-                        const int ih = (SI + in) >> 1, im = SI - in;
-                        const int jh = (SJ + jn) >> 1, jm = SJ - jn;
-                        const int kh = (SK + kn) >> 1, km = SK - kn;
-                        const bool valid = ok && ih < nu && im >= 0 && jh < nv && jm >= 0 && kh < nw && km >= 0;
-                        if (valid) {
-                            const int il = im >> 1, jl = jm >> 1, kl = km >> 1;
-                            const int Ih = ih * nyz, Il = il * nyz, Jh = jh * rw, Jl = jl * rw;
-                            const double t4 = fs[Ih + Jh + kh] * fs[Il + Jl + kl] + fs[Ih + Jl + kl] * fs[Il + Jh + kh] +
-                                              fs[Ih + Jh + kl] * fs[Il + Jl + kh] + fs[Ih + Jl + kh] * fs[Il + Jh + kl];
+                        if (in <= LI && jn <= LJ && kn <= LK) {
+                            const int qi = in * nyz, qj = jn * rw;
+                            const int hhh = (P + qi + qj + kn) >> 1, lll = (P - qi - qj - kn) >> 1;
+                            const int hll = (P + qi - qj - kn) >> 1, lhh = (P - qi + qj + kn) >> 1;
+                            const int hhl = (P + qi + qj - kn) >> 1, llh = (P - qi - qj + kn) >> 1;
+                            const int hlh = (P + qi - qj + kn) >> 1, lhl = (P - qi + qj - kn) >> 1;
+                            const double t4 = fs[hhh] * fs[lll] + fs[hll] * fs[lhh] + fs[hhl] * fs[llh] +
+                                              fs[hlh] * fs[lhl];
                             acc += val * t4;
                         }
                     }
@@ -219,7 +233,7 @@ int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells
     const size_t lds = sizeof(double) * nu * nv * gain_row(nw);
     GDT_CHECK(lds <= 160 * 1024, GDT_E_ARG, "gain: grid of %d box nodes exceeds the LDS budget", h->n_box);
     KernelTimer kt(h, "gain");
-    hipLaunchKernelGGL(k_gain, dim3((blocks + 3) / 4, cells), dim3(256), lds, h->stream, nu, nv, nw,
+    hipLaunchKernelGGL(k_gain, dim3((blocks + 7) / 8, cells), dim3(512), lds, h->stream, nu, nv, nw,
                        h->gptr.as<int>(), h->gbox.as<int>(), h->gval.as<double>(), f_box, gain_box);
     GDT_HIP(hipGetLastError());
     return GDT_OK;
