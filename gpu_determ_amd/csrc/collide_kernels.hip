@@ -206,6 +206,10 @@ __global__ __launch_bounds__(512) void k_gain(int nu, int nv, int nw, const int 
                     const int LI = ok ? min(SI, 2 * nu - 1 - SI) : -1;
                     const int LJ = min(SJ, 2 * nv - 1 - SJ), LK = min(SK, 2 * nw - 1 - SK);
                     const int P = SI * nyz + SJ * rw + SK;
+                    // unrolled by 4: the scalar loads of four entries go out
+                    // together (508 -> 535 evaluations per second at 16^3,
+                    // 70 -> 77 at 20^3; by 2: 529, 73)
+#pragma unroll 4
                     for (int e = e0; e < e1; e++) {
                         const int box = gbox[e];
                         const double val = gval[e];
