@@ -100,13 +100,17 @@ int gdt_basis(gdt_handle h, double *dst);
 /* ---- analytic collision matrix (init_matrices, compute_matrix.cu:564-600) */
 /* Builds b[n_box] and a[n_box*n_box/8] (reference index order) on the GPU in
  * FP64, and the per-relative-velocity lists of nonzero boxes.  mode is
- * GDT_TABLES_FP64 or GDT_TABLES_REFERENCE.  Requires even n on every axis
- * and a box of at most 24^3 nodes. */
+ * GDT_TABLES_FP64 or GDT_TABLES_REFERENCE.  Requires even n of at most
+ * 1,024 on every axis and a box of at most 32,768 nodes.  The gain holds f of
+ * a cell in LDS, rows padded to n_w rounded up to 4 mod 8 doubles: the matrix
+ * evaluation takes boxes with n_u n_v rw 8 B <= 160 KiB (26^3 at most for a
+ * cube); larger ones return GDT_E_ARG there. */
 int gdt_build_tables(gdt_handle h, int mode);
 /* Device pointers to the FP64 tables owned by the handle. */
 int gdt_tables(gdt_handle h, const double **b, const double **a, long *n_a, long *nnz);
 /* Replace the handle's tables by caller-provided ones (device FP64, same
- * layout), e.g. tables imported from FP32 reference arrays. */
+ * layout), e.g. tables imported from FP32 reference arrays.  Same axis limits
+ * as gdt_build_tables. */
 int gdt_set_tables(gdt_handle h, const double *b, const double *a);
 
 /* ---- Korobov lattice stencil (no reference counterpart) ---------------- */
