@@ -68,6 +68,22 @@ def test_loss_gain_8cube():
     assert rel_l2(gain, g.to_cmp(og)) < 1e-13
 
 
+@pytest.mark.parametrize("n", [(8, 6, 10), (6, 12, 4)])
+def test_loss_gain_non_cubic(n):
+    """Unequal axes (row padding of 2 and 0, per-axis bounds of the gain) against the oracle."""
+    say("loss/gain", n)
+    g = O.Grid(n, (-4.0, -3.5, -4.5), (4.0, 3.5, 4.5), -1.0)
+    c = ctx_for(g)
+    c.build_tables(0)
+    ob, oa = O.build_tables(g, 0)
+    rng = np.random.default_rng(11)
+    f = np.stack([mixture(g.xi, rng) for _ in range(2)])
+    nu, gain = c.collide_matrix(G.DeviceArray.from_host(f), 2)
+    nu, gain = nu.get(), gain.get()
+    assert rel_l2(nu, g.to_cmp(O.loss(g, ob, g.to_box(f)))) < 1e-13
+    assert rel_l2(gain, g.to_cmp(O.gain(g, oa, g.to_box(f)))) < 1e-13
+
+
 def test_set_tables_lists_match_built():
     """gdt_set_tables derives the nonzero-box lists on the host; with the
     GPU-built tables of another handle, its gain equals that handle's bit for
