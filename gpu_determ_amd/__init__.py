@@ -241,6 +241,19 @@ class Context:
         check(lib().gdt_build_tables(self.h, int(mode)))
         self.sync()
 
+    def set_tables(self, b, a):
+        """Replaces the tables by device FP64 arrays b [n_box] and a
+        [n_box^2 / 8] in the reference's index order."""
+        check(lib().gdt_set_tables(self.h, b.ptr, a.ptr))
+        self.sync()
+
+    def basis(self):
+        """The 14 moment-basis planes over the box, host FP64 [14][n_box]."""
+        out = DeviceArray((14, self.n_box))
+        check(lib().gdt_basis(self.h, out.ptr))
+        self.sync()
+        return out.get()
+
     def tables(self):
         """Host copies of (b, a) and the nonzero count."""
         b, a = C.c_void_p(), C.c_void_p()

@@ -230,12 +230,26 @@ __global__ __launch_bounds__(512) void k_gain(int nu, int nv, int nw, const int 
     if (act) out[(size_t)c * nb + (i * nv + j) * nw + k] = acc;
 }
 
+// Enables k_gain for the LDS copy of f of the context's box (above 64 KiB
+// the kernel must be enabled for its dynamic LDS).  Called when the tables
+// are built, outside any stream capture; launch_gain checks it again.
+int gain_prepare(gdt_context *h)
+{
+    const size_t lds = sizeof(double) * h->n[0] * h->n[1] * gain_row(h->n[2]);
+    if (lds > 160 * 1024 || h->gain_lds_attr >= (int)lds) return GDT_OK;  // launch_gain rejects the former
+    GDT_HIP(hipFuncSetAttribute((const void *)k_gain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    h->gain_lds_attr = (int)lds;
+    return GDT_OK;
+}
+
 int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells)
 {
     const int nu = h->n[0], nv = h->n[1], nw = h->n[2];
     const int blocks = ((nu + 3) / 4) * ((nv + 3) / 4) * ((nw + 3) / 4);
     const size_t lds = sizeof(double) * nu * nv * gain_row(nw);
     GDT_CHECK(lds <= 160 * 1024, GDT_E_ARG, "gain: grid of %d box nodes exceeds the LDS budget", h->n_box);
+    int rc = gain_prepare(h);
+    if (rc) return rc;
     KernelTimer kt(h, "gain");
     hipLaunchKernelGGL(k_gain, dim3((blocks + 7) / 8, cells), dim3(512), lds, h->stream, nu, nv, nw,
                        h->gptr.as<int>(), h->gbox.as<int>(), h->gval.as<double>(), f_box, gain_box);

@@ -1,5 +1,6 @@
 // C ABI of libgdt.so (declarations and contracts: include/gdt.h).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -22,6 +23,10 @@ void gdt_set_error(int code, const char *fmt, ...)
     g_last_error = std::string(buf) + " [status " + std::to_string(code) + "]";
 }
 
+static std::atomic<unsigned long> g_devbuf_gen{0};
+
+unsigned long devbuf_generation() { return g_devbuf_gen.load(std::memory_order_acquire); }
+
 int DevBuf::ensure(size_t need)
 {
     if (need <= bytes && p) return 0;
@@ -39,7 +44,10 @@ int DevBuf::ensure(size_t need)
 
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p) {
+        (void)hipFree(p);
+        g_devbuf_gen.fetch_add(1, std::memory_order_acq_rel);
+    }
     p = nullptr;
     bytes = 0;
 }
@@ -372,8 +380,10 @@ int gdt_build_tables(gdt_handle h, int mode)
     for (int a = 0; a < 3; a++)
         GDT_CHECK(h->n[a] <= GLIST_AXIS_MAX, GDT_E_ARG, "gdt_build_tables: n[%d] = %d (max %d)", a, h->n[a],
                   GLIST_AXIS_MAX);
-    const int rc = launch_build_tables(h, mode);
+    h->mx_ready = false;  // the matrix-core operands are built from the tables
+    int rc = launch_build_tables(h, mode);
     if (rc != GDT_OK) return rc;
+    if ((rc = gain_prepare(h)) != GDT_OK) return rc;
     h->table_mode = mode;
     h->have_tables = true;
     return GDT_OK;
@@ -400,6 +410,7 @@ int gdt_set_tables(gdt_handle h, const double *b, const double *a)
                   "gdt_set_tables: n[%d] = %d must be even and at most %d", ax, h->n[ax], GLIST_AXIS_MAX);
     const int nb = h->n_box;
     const long n_a = (long)nb * nb / 8;
+    h->mx_ready = false;  // the matrix-core operands are built from the tables
     GDT_CHECK(h->tb.ensure(sizeof(double) * nb) == 0 && h->ta.ensure(sizeof(double) * n_a) == 0, GDT_E_NOMEM,
               "gdt_set_tables: allocation failed");
     GDT_HIP(hipMemcpyAsync(h->tb.p, b, sizeof(double) * nb, hipMemcpyDeviceToDevice, h->stream));
@@ -437,6 +448,8 @@ int gdt_set_tables(gdt_handle h, const double *b, const double *a)
     }
     h->n_a = n_a;
     h->n_gent = (long)gb.size();
+    const int rc = gain_prepare(h);
+    if (rc != GDT_OK) return rc;
     h->table_mode = -1;
     h->have_tables = true;
     return GDT_OK;
@@ -1014,8 +1027,12 @@ int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int
         }
     if (chunk < 8 && steps > 64) chunk = 64;
     const unsigned key = flags & (GDT_FLAG_CORRECT | GDT_FLAG_FUSED);
+    // the captured kernels hold raw pointers to scratch and table buffers:
+    // any DevBuf freed since the capture (a larger call elsewhere, a table
+    // rebuild) makes the graph stale
+    const unsigned long gen = devbuf_generation();
     if (!h->graph_exec || h->g_method != method || h->g_cells != cells || h->g_dt != dt || h->g_flags != key ||
-        h->g_f != f || h->g_chunk != chunk) {
+        h->g_f != f || h->g_chunk != chunk || h->g_gen != gen) {
         if (h->graph_exec) {
             (void)hipGraphExecDestroy(h->graph_exec);
             h->graph_exec = nullptr;
@@ -1038,6 +1055,7 @@ int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int
         h->g_flags = key;
         h->g_f = f;
         h->g_chunk = chunk;
+        h->g_gen = devbuf_generation();
     }
     const int full = steps / chunk, rem = steps % chunk;
     for (int s = 0; s < full; s++) GDT_HIP(hipGraphLaunch(h->graph_exec, h->stream));

@@ -47,6 +47,10 @@ struct DevBuf {
     void release();
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
+// Process-wide count of DevBuf frees (growth or release).  A captured graph
+// holds raw pointers to context buffers, so it is replayed only while this
+// count is the one read at its capture.
+unsigned long devbuf_generation();
 
 struct KernelStat {
     double ms = 0.0;
@@ -92,6 +96,7 @@ struct gdt_context {
     std::vector<int32_t> h_kptr;  // host copy of the CSR row pointers
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
     int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
+    int gain_lds_attr = 0;        // dynamic-LDS bytes the matrix gain kernel is enabled for
 
     // batched matrix form on the matrix cores (16^3 box): operands built from
     // the tables (B = Toeplitz b, AwT = a by parity class; bf16 hi / lo, fp32)
@@ -107,6 +112,7 @@ struct gdt_context {
     double g_dt = 0.0;
     unsigned g_flags = 0;
     double *g_f = nullptr;
+    unsigned long g_gen = 0;  // devbuf_generation() at capture
 
     // timing
     bool timing = false;
@@ -175,6 +181,7 @@ uint32_t host_korobov_pick(uint32_t M, uint64_t seed, int ncand, double *p2);
 // ---- collision kernels (collide_kernels.hip) ------------------------------
 int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells);
 int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells);
+int gain_prepare(gdt_context *h);
 int launch_cmp_to_box(gdt_context *h, const double *f_cmp, double *f_box, int cells, int layout);
 int launch_box_to_cmp(gdt_context *h, const double *f_box, double *f_cmp, int cells, int layout);
 // ---- matrix cores (mfma_gemm.hip, matrix_mfma.hip) ---------------------------

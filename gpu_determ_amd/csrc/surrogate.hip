@@ -279,10 +279,15 @@ __global__ __launch_bounds__(256) void k_sg_residual(int n, const double *__rest
         const double D = (part[0][0] + part[1][0]) + (part[2][0] + part[3][0]);
         const double S = (part[0][1] + part[1][1]) + (part[2][1] + part[3][1]);
         const double E = (part[0][2] + part[1][2]) + (part[2][2] + part[3][2]);
-        double r = S > 0.0 ? sqrt(D / S) : 0.0;
+        // fails closed: a NaN or Inf anywhere in the cell's sums (a zero
+        // density, an overflow in the network), or a zero Q against a
+        // nonzero prediction, sends the cell to the plain solver
+        double r = S > 0.0 ? sqrt(D / S) : (D > 0.0 ? 1.0 : 0.0);
         if (E > 0.0) r = 1.0;
+        const bool finite = isfinite(D) && isfinite(S) && isfinite(E) && isfinite(r);
+        if (!finite) r = INFINITY;
         res[blockIdx.x] = r;
-        flag[blockIdx.x] = r > thr ? 1 : 0;
+        flag[blockIdx.x] = (finite && r <= thr) ? 0 : 1;
     }
 }
 

@@ -139,3 +139,39 @@ def test_korobov_set_tiles_and_batch_layout_check():
     assert np.array_equal(Qc3.get(), a)
     with pytest.raises(RuntimeError):
         ctx.korobov_set_tiles(0)
+
+
+def test_batch_uneven_grids_vs_oracle():
+    """The uneven-batch configuration's grid classes, {16, 20, 24, 32}^3 with
+    2^20 lattice nodes each: a batched evaluation of every cell matches the
+    FP64 oracle (stencil, collide and correction on the CPU) to 1e-12, and a
+    batched explicit step equals single-cell steps bit for bit."""
+    import oracle as O
+    from helpers import rel_l2
+    M = 1 << 20
+    sizes = (16, 20, 24, 32)
+    cells = [3, 2, 2, 2]
+    print("batch {16,20,24,32}^3 at 2^20 vs oracle", flush=True)
+    classes = make_classes(sizes, M=M, seed=7)
+    rng = np.random.default_rng(17)
+    f0 = [cell_fs(c, k, rng) for c, k in zip(classes, cells)]
+    fd = [G.DeviceArray.from_host(x) for x in f0]
+    qd = [G.DeviceArray((k, c.n_pnt)) for c, k in zip(classes, cells)]
+    b = G.Batch(classes, fd, qd)
+    b.config(4, 8)
+    b.run(0.0, 1, G.FLAG_CORRECT)  # evaluation: Q out, f unchanged
+    b.sync()
+    for n, ctx, x0, q in zip(sizes, classes, f0, qd):
+        g = O.Grid((n,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+        st = O.korobov_stencil(g, M, ctx.korobov_info()["a"], 0.25)
+        lo, q0 = O.korobov_collide(g, st, x0)
+        oq, _, bad = O.correct_loss(g.xi, lo, q0, x0.shape[0])
+        assert bad == 0
+        err = rel_l2(q.get(), oq)
+        print("  %d^3: %d cells, rel L2 vs oracle %.3g" % (n, x0.shape[0], err), flush=True)
+        assert err < 1e-12
+    b.run(DT, 1, G.FLAG_CORRECT)
+    b.sync()
+    for ctx, x0, d in zip(classes, f0, fd):
+        assert np.array_equal(d.get(), single_runs(ctx, x0, 1, G.FLAG_CORRECT))
+    b.close()

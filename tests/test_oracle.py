@@ -394,3 +394,173 @@ def test_korobov_correction_conserves(kor20):
     Qc, X, bad = O.correct_loss(g.xi, lo, Q, 1)
     assert bad == 0
     assert np.max(np.abs(moments(g.xi, Qc, g.d3v)) / np.abs(scale)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# Independent pins of the Korobov generator's figure of merit and choice, and
+# of the stencil's selection rule
+# ---------------------------------------------------------------------------
+def p2_dual(M, a, dims):
+    """P_2 of the rank-1 lattice z = (1, a, a^2, ...) mod M through its dual
+    lattice, not through the point sum the oracle evaluates.
+
+    1 + 2 pi^2 B_2(x) = sum_h e^(2 pi i h x) / r(h)^2 with r(h) = max(1, |h|)
+    (the Fourier series of B_2), so P_2 = sum over h != 0 with
+    sum_j h_j z_j = 0 (mod M) of prod_j r(h_j)^-2.  The sum of r(h)^-2 over a
+    residue class c mod M is closed: S(c) = pi^2 / (M sin(pi c / M))^2 for
+    c != 0 (the cosecant-square series) and 1 + pi^2 / (3 M^2) for c = 0
+    (zeta(2)).  The residues of h_1 .. h_{d-1} fix the class of h_0, so
+    P_2 + 1 = sum_c S(-c) T(c), T the cyclic convolution over j >= 1 of
+    u_j(c) = sum_{rho : z_j rho = c} S(rho)."""
+    c = np.arange(M)
+    S = np.empty(M)
+    S[0] = 1.0 + math.pi ** 2 / (3.0 * M * M)
+    S[1:] = (math.pi / (M * np.sin(math.pi * c[1:] / M))) ** 2
+    z = [1]
+    for _ in range(dims - 1):
+        z.append(z[-1] * a % M)
+    T = np.zeros(M)
+    T[0] = 1.0
+    for j in range(1, dims):
+        u = np.zeros(M)
+        np.add.at(u, (z[j] * c) % M, S)
+        T = u[(c[:, None] - c[None, :]) % M] @ T
+    return float(S[(-c) % M] @ T - 1.0)
+
+
+@pytest.mark.parametrize("M", [16, 64, 256, 1024])
+def test_korobov_p2_one_dim_closed_form(M):
+    """dims = 1: the multiplication theorem sum_k B_2(k / M) = B_2(0) / M gives
+    P_2 = pi^2 / (3 M^2) exactly; the oracle's point sum cancels down to it
+    (absolute rounding ~1e-15)."""
+    assert abs(O.korobov_p2(M, 1, 1) - math.pi ** 2 / (3.0 * M * M)) < 1e-13
+    assert abs(p2_dual(M, 1, 1) - math.pi ** 2 / (3.0 * M * M)) < 1e-15
+
+
+@pytest.mark.parametrize("M,a,dims", [(16, 5, 2), (64, 19, 2), (64, 1, 2), (256, 77, 2), (64, 13, 3),
+                                      (128, 29, 4), (256, 3, 8), (1024, 389, 8), (1024, 1, 8)])
+def test_korobov_p2_matches_dual_lattice(M, a, dims):
+    """The oracle's P_2 (a sum over the M lattice points) equals the dual
+    lattice sum in closed form.  A dropped 1/6 in B_2, a wrong 2 pi^2 or a
+    wrong generator power moves it by far more than the rounding allowed."""
+    got = O.korobov_p2(M, a, dims)
+    ref = p2_dual(M, a, dims)
+    assert abs(got - ref) <= 1e-9 * (1.0 + abs(ref)), (got, ref)
+
+
+# splitmix64's published first outputs for seed 0 (Steele, Lea and Flood,
+# "Fast splittable pseudorandom number generators", 2014; Vigna's reference
+# splitmix64.c)
+SPLITMIX0 = (0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F)
+
+
+def test_korobov_pick_known_answer():
+    """Generator choice: candidate k is 2 (x_k mod M/2) + 1 for the k-th
+    splitmix64 output x_k of the seed; the one with the least P_2 wins.  Pinned
+    with the published splitmix64 vector and the dual-lattice P_2."""
+    M = 1 << 10
+    cands = [max(3, (x % (M // 2)) * 2 + 1) for x in SPLITMIX0]
+    a1, p1 = O.korobov_pick(M, 0, 1)
+    assert a1 == cands[0]
+    assert abs(p1 - p2_dual(M, a1, 8)) <= 1e-9 * (1 + p1)
+    a3, _ = O.korobov_pick(M, 0, 3)
+    p = [p2_dual(M, c, 8) for c in cands]
+    assert a3 == cands[int(np.argmin(p))]
+
+
+def stencil_rule(g, M, a):
+    """The two-point projection stencil restated in numpy from its rule, with
+    numpy's sin / cos: lattice point x = nu z mod M in 8 dimensions; alpha,
+    beta the grid nodes under (x_0..2, x_3..5) n / M; the impact direction from
+    (x_6, x_7); xp = (alpha + beta) / 2 + |alpha - beta| / 2 omega the exact
+    post-collision velocity.  Returns per lattice point: kept flag, alpha,
+    beta (compact), lambda / mu (box coordinates), delta = xp - lambda, the
+    energies E0 and E_lm, and the in-domain flags of every candidate second
+    pair (lambda + s, mu - s), s_k in {0, sign delta_k}, with their energies
+    and squared distances to xp."""
+    n = np.array(g.n)
+    z = [1]
+    for _ in range(7):
+        z.append(z[-1] * a % M)
+    nu = np.arange(M, dtype=np.int64)
+    x = np.stack([(nu * zj) % M for zj in z], axis=1)
+    ia = x[:, 0:3] * n // M
+    ib = x[:, 3:6] * n // M
+    box2c = g.box2c
+
+    def comp(p):
+        ok = np.all((p >= 0) & (p < n), axis=1)
+        q = np.clip(p, 0, n - 1)
+        c = box2c[(q[:, 0] * n[1] + q[:, 1]) * n[2] + q[:, 2]]
+        return np.where(ok, c, -1)
+
+    def energy(p):
+        return np.sum((2 * p + 1 - n) ** 2, axis=1)
+
+    ca, cb = comp(ia), comp(ib)
+    D2 = np.sum((ia - ib) ** 2, axis=1)
+    t7 = (x[:, 6] + 0.5) / M
+    t8 = (x[:, 7] + 0.5) / M
+    rho = 2.0 * np.sqrt(t7 * (1.0 - t7))
+    om = np.stack([rho * np.cos(2 * np.pi * t8), rho * np.sin(2 * np.pi * t8), 1.0 - 2.0 * t7], axis=1)
+    xp = 0.5 * (ia + ib) + 0.5 * np.sqrt(D2)[:, None] * om
+    lam = np.floor(xp + 0.5).astype(np.int64)
+    mu = ia + ib - lam
+    dl = xp - lam
+    E0 = energy(ia) + energy(ib)
+    El = energy(lam) + energy(mu)
+    sg = np.where(dl >= 0, 1, -1)
+    cand = []
+    for S in range(1, 8):
+        m = np.array([(S >> k) & 1 for k in range(3)])
+        s = sg * m
+        ls, ms = lam + s, mu - s
+        inside = (comp(ls) >= 0) & (comp(ms) >= 0)
+        Es = energy(ls) + energy(ms)
+        brk = np.where(El < E0, Es >= E0, Es <= E0)
+        d2 = np.sum(np.where(m == 1, (1.0 - np.abs(dl)) ** 2, dl ** 2), axis=1)
+        cand.append((s, inside & brk, Es, d2))
+    kept = (ca >= 0) & (cb >= 0) & (D2 > 0) & (comp(lam) >= 0) & (comp(mu) >= 0)
+    kept &= (El == E0) | np.any(np.stack([c[1] for c in cand]), axis=0)
+    return dict(kept=kept, ca=ca, cb=cb, lam=lam, mu=mu, dl=dl, E0=E0, El=El, cand=cand)
+
+
+@pytest.mark.parametrize("n,R,M,seed", [((20,) * 3, 4.8, 1 << 16, 5), ((12,) * 3, -1.0, 1 << 14, 2)])
+def test_korobov_stencil_selection_rule(n, R, M, seed):
+    """Every stencil entry of the oracle obeys the projection rule as numpy
+    restates it: the kept lattice points are exactly those whose pair, nearest
+    node pair and some bracketing second pair lie in the grid; lambda is the
+    nearest node of xp on every axis (|delta_k| <= 1/2); mu = alpha + beta -
+    lambda; r = 0 exactly when E_lm = E0; otherwise (lambda + s, mu - s) has
+    s_k in {0, sign delta_k}, brackets E0 from the other side, has the least
+    squared distance to xp among the in-domain bracketing candidates, and
+    r = (E0 - E_lm) / (E_s - E_lm).  Taking floor for the rounding, or the
+    largest distance, fails it."""
+    vm = 4.8
+    g = O.Grid(n, (-vm,) * 3, (vm,) * 3, R)
+    a, _ = O.korobov_pick(M, seed, 8)
+    idx6, r, w, node = O.korobov_stencil(g, M, a, 0.25)
+    ru = stencil_rule(g, M, a)
+    assert np.array_equal(np.nonzero(ru["kept"])[0], node.astype(np.int64))
+    t = node.astype(np.int64)
+    assert np.array_equal(idx6[:, 0], ru["ca"][t]) and np.array_equal(idx6[:, 1], ru["cb"][t])
+    c = g.cidx.astype(np.int64)
+    lam, mu, dl = ru["lam"][t], ru["mu"][t], ru["dl"][t]
+    assert np.all(np.abs(dl) <= 0.5 + 1e-12)
+    assert np.array_equal(c[idx6[:, 2]], lam) and np.array_equal(c[idx6[:, 3]], mu)
+    E0, El = ru["E0"][t], ru["El"][t]
+    assert np.array_equal(r == 0.0, El == E0)
+    nz = np.nonzero(r != 0.0)[0]
+    s = c[idx6[nz, 4]] - lam[nz]
+    assert np.array_equal(c[idx6[nz, 5]], mu[nz] - s)
+    sg = np.where(dl[nz] >= 0, 1, -1)
+    assert np.all((s == 0) | (s == sg)) and np.all(np.any(s != 0, axis=1))
+    P = 2 * c + 1 - np.array(g.n)
+    Es = np.sum(P[idx6[nz, 4]] ** 2, axis=1) + np.sum(P[idx6[nz, 5]] ** 2, axis=1)
+    assert np.all((Es - E0[nz]) * (El[nz] - E0[nz]) <= 0)
+    assert np.array_equal(r[nz], (E0[nz] - El[nz]) / (Es - El[nz]))
+    d2 = np.sum(np.where(s != 0, (1.0 - np.abs(dl[nz])) ** 2, dl[nz] ** 2), axis=1)
+    best = np.full(len(nz), np.inf)
+    for sc, ok, _, cd2 in ru["cand"]:
+        best = np.where(ok[t[nz]], np.minimum(best, cd2[t[nz]]), best)
+    assert np.all(d2 <= best + 1e-12)
