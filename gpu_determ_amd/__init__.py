@@ -27,6 +27,8 @@ FLAG_GRAPH = 2
 FLAG_TIME = 4
 FLAG_COURANT = 8
 FLAG_FUSED = 16
+FLAG_CELL_TILES = 32
+FLAG_MIXED = 64
 
 GEMM_BF16 = 0
 GEMM_BF16_SPLIT = 1
@@ -47,7 +49,7 @@ ABI_SYMBOLS = (
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
     "gdt_init_mixture", "gdt_korobov_tiles", "gdt_korobov_set_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
-    "gdt_transport",
+    "gdt_transport", "gdt_korobov_cell_tiles",
 )
 
 _lib = None
@@ -130,6 +132,7 @@ def lib():
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
         "gdt_korobov_tiles": [vp, ip, ip, ip],
         "gdt_korobov_set_tiles": [vp, C.c_int],
+        "gdt_korobov_cell_tiles": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "gdt_transport": [vp, vp, vp, C.c_int, C.c_int, C.c_int, dp, C.c_double, ip],
         "gdt_kmath_pow": [vp, vp, vp, vp, C.c_long],
         "gdt_kmath_tables": [vp],
@@ -285,6 +288,12 @@ class Context:
         """Limit of gather tiles per node group (default 64); rebuilds the
         tiles of a built stencil."""
         check(lib().gdt_korobov_set_tiles(self.h, int(tiles_max)))
+
+    def korobov_cell_tiles(self, chunk_tiles=4, node_blocks=2048, gather_blocks=2048, wide=0):
+        """Launch shape of the cell-tile paths (chunk of tiles, grid sizes,
+        tile width 16 / 32 or 64 cells)."""
+        check(lib().gdt_korobov_cell_tiles(self.h, int(chunk_tiles), int(node_blocks), int(gather_blocks),
+                                           int(wide)))
 
     def kmath_pow(self, x, r):
         """x**r through the node terms' table-driven log / exp (FP64 device

@@ -1,0 +1,494 @@
+// Many-cell Korobov operator in cell tiles (gfx950, wave64).
+//
+// The single-cell path reads every stencil record and entry once per cell and
+// gathers 16-byte node values one request per entry.  Here K cells share each
+// of those reads: a tile holds K cells interleaved [node][K], so one 128-byte
+// row carries a node's value for all K cells (K = 16 in FP64, 32 in FP32), and
+// one stencil record or entry serves K lanes at the same address.
+//
+// Node-term type T: double (the plain method: the same operations as the
+// single-cell node kernel, so the node values are the same bits) or float
+// (the mixed form: f, the node terms D and 2 w P in FP32 with the hardware's
+// log2 / exp2; every sum and the correction in FP64).
+//
+// Per evaluation of C cells in T = ceil(C / K) tiles (cells past C are zero):
+//   k_ct_pack     f [cell][node] FP64  ->  ft[tile][node][K] of T
+//   k_ct_node     per chunk of tiles: D of every (node, cell) into
+//                 D[tile][p][K] (p: the sorted node position)
+//   k_ct_gather   per chunk: a group of K lanes per destination walks the
+//                 destination's entries of the global destination-sorted CSR
+//                 in order (one FP64 sum per cell, no partials over node
+//                 groups): Q0[tile][d][K] = sum coef D and the loss
+//                 L[tile][d][K] = f_d sum 2 w f_partner over the entries where
+//                 d is alpha or beta (the loss is linear in f given f_d, so it
+//                 reads the f tile, which stays in L2, instead of a stored
+//                 2 w P); destinations are taken longest first, so the lane
+//                 groups of a wave have similar entry counts
+//   k_ct_moments  per (destination range, tile): the 20 moment sums of the
+//                 range, one partial per cell and range
+//   k_ct_solve    per cell: the partials in a fixed order, the 5x5 solve: X
+//   k_ct_apply    per (destination range, tile): Q = Q0 - L X.psi, loss,
+//                 corrected nu, f += dt Q, written cell-major through an LDS
+//                 transpose
+// No atomics: every sum has a fixed order, so results are reproducible run to
+// run.  Chunks of tiles bound the node values in flight between the node and
+// gather kernels (ct_chunk_tiles tiles: about 200 MB at 32^3 and 2^21 nodes,
+// so a chunk's node values can stay in the 256 MB Infinity Cache).
+#include <algorithm>
+
+#include "gdt_internal.h"
+#include "kernel_common.h"
+#include "kmath.h"
+
+#define CT_THREADS 256
+#define CT_MOM_PARTS 64   // destination ranges (moment partials) per tile
+#define CT_UNROLL 8       // entries whose loads a lane issues together
+
+template <class T> struct CtK;
+template <> struct CtK<double> {
+    static constexpr int K = 16;
+};
+template <> struct CtK<float> {
+    static constexpr int K = 32;
+};
+
+// base[idx] with a 32-bit byte offset: a uniform base and a per-lane 32-bit
+// offset (global_load ... saddr), no 64-bit address arithmetic per lane.  The
+// scratch the offsets index stays below 4 GiB (checked by ct_prepare).
+template <class T>
+__device__ __forceinline__ T ld_off(const T *__restrict__ base, uint32_t idx)
+{
+    return *(const T *)((const char *)base + (size_t)(uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
+// ---------------------------------------------------------------------------
+// f tiles
+// ---------------------------------------------------------------------------
+// Block (x, t): nodes [64 x, 64 x + 64) of tile t, through an LDS tile: rows of
+// 64 contiguous nodes in, rows of K contiguous cells out.
+template <class T, int K>
+__global__ __launch_bounds__(CT_THREADS) void k_ct_pack(int npts, int cells, const double *__restrict__ f,
+                                                        T *__restrict__ ft)
+{
+    __shared__ T tile[K][65];
+    const int v0 = blockIdx.x * 64, t = blockIdx.y;
+    const int lane = threadIdx.x % 64;
+    for (int k = threadIdx.x / 64; k < K; k += CT_THREADS / 64) {
+        const long c = (long)t * K + k;
+        const int v = v0 + lane;
+        tile[k][lane] = (c < cells && v < npts) ? (T)f[c * npts + v] : (T)0;
+    }
+    __syncthreads();
+    T *out = ft + ((size_t)t * npts + v0) * K;
+    for (int i = threadIdx.x; i < 64 * K; i += CT_THREADS) {
+        const int vl = i / K, k = i % K;
+        if (v0 + vl < npts) out[i] = tile[k][vl];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// node terms
+// ---------------------------------------------------------------------------
+// Packed node record x = (a | b << 15 | D2 >> 10 << 30, l | m << 15,
+// ls | ms << 15, num / 4 | den / 4 << 11 | (D2 & 1023) << 22); fr the tile's
+// f rows, k the lane's cell.
+__device__ __forceinline__ int ct_num(uint32_t w) { return (int)(w & 0x7FFu); }
+__device__ __forceinline__ int ct_den(uint32_t w) { return (int)((w >> 11) & 0x7FFu); }
+__device__ __forceinline__ uint32_t ct_d2(const uint4 &x) { return (x.w >> 22) | ((x.x >> 30) << 10); }
+
+// FP64: the operations of the single-cell node kernel (table-driven log and
+// exp from ts, contraction off), so D is the same bits.  ld(i) is the lane's
+// f value of grid node i.
+template <class LD>
+__device__ __forceinline__ double ct_terms(const uint4 x, LD ld, const double *ts, double Kw)
+{
+#pragma clang fp contract(off)
+    const int num = ct_num(x.w), den = ct_den(x.w);
+    const double fa = ld(x.x & 0x7FFFu), fb = ld((x.x >> 15) & 0x7FFFu);
+    const double fl = ld(x.y & 0x7FFFu), fm = ld(x.y >> 15);
+    const double fls = ld(x.z & 0x7FFFu), fms = ld(x.z >> 15);
+    const double P = fa * fb;
+    const double Pl = fl * fm;
+    double R;
+    if (num == 0) {
+        R = Pl;
+    } else {
+        const double r = (double)num / (double)den;
+        const double Ps = fls * fms;
+        if (Pl > 0.0 && Ps > 0.0)
+            R = Pl * kmt_exp(r * kmt_log(Ps / Pl, ts), ts);
+        else
+            R = (1.0 - r) * Pl + r * Ps;
+    }
+    const double w = Kw * sqrt((double)ct_d2(x));
+    return w * (P - R);
+}
+
+// FP32: the same formula in single precision with the hardware's
+// transcendentals.  The ratio Ps / Pl is split by frexp into mantissas (whose
+// quotient lies in (1/2, 2): v_log_f32 never sees a denormal) and an exact
+// integer exponent difference; the quotient is one reciprocal refined by a
+// Newton step (within an ulp of the rounded quotient); r = num / den by the
+// hardware reciprocal (den <= 2047).  R = Pl 2^(r log2(Ps / Pl)); an exponent
+// below -126 gives 0 (a value below 2^-126 Pl, far below the sums' rounding).
+// Products that are not positive take the linear form, as in FP64.
+template <class LD>
+__device__ __forceinline__ float ct_terms(const uint4 x, LD ld, const double *ts, float Kw)
+{
+    (void)ts;
+    const int num = ct_num(x.w), den = ct_den(x.w);
+    const float fa = ld(x.x & 0x7FFFu), fb = ld((x.x >> 15) & 0x7FFFu);
+    const float fl = ld(x.y & 0x7FFFu), fm = ld(x.y >> 15);
+    const float fls = ld(x.z & 0x7FFFu), fms = ld(x.z >> 15);
+    const float P = fa * fb;
+    const float Pl = fl * fm, Ps = fls * fms;
+    const float r = (float)num * __builtin_amdgcn_rcpf((float)den);
+    float R = Pl;
+    if (num != 0) {
+        if (Pl > 0.0f && Ps > 0.0f) {
+            const float ms = __builtin_amdgcn_frexp_mantf(Ps), ml = __builtin_amdgcn_frexp_mantf(Pl);
+            const int es = __builtin_amdgcn_frexp_expf(Ps), el = __builtin_amdgcn_frexp_expf(Pl);
+            const float rc = __builtin_amdgcn_rcpf(ml);
+            float qm = ms * rc;
+            qm = fmaf(fmaf(-ml, qm, ms), rc, qm);
+            const float lg = __builtin_amdgcn_logf(qm) + (float)(es - el);
+            R = Pl * __builtin_amdgcn_exp2f(r * lg);
+        } else {
+            R = (1.0f - r) * Pl + r * Ps;
+        }
+    }
+    const float w = Kw * __builtin_amdgcn_sqrtf((float)ct_d2(x));
+    return w * (P - R);
+}
+
+// Lane group of K lanes: its group index, made uniform (an SGPR) when the
+// group is the whole wave, so that records, node rows and their addresses
+// are scalar.
+template <int K>
+__device__ __forceinline__ int ct_group()
+{
+    if (K == WAVE) return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    return (int)(threadIdx.x / K);
+}
+
+// Lane k's value of grid node i in the rows of `fr` (K values per row): a row
+// pointer when the group is the wave (uniform i: a scalar base, one lane
+// offset), a 32-bit offset load otherwise.
+template <int K, class T>
+__device__ __forceinline__ T ct_row(const T *__restrict__ fr, uint32_t i, int k)
+{
+    if (K == WAVE) return (fr + (size_t)i * K)[k];
+    return ld_off(fr, i * K + k);
+}
+
+// Block (x, j) of a chunk: tile t0 + j; lane group g = tid / K takes nodes
+// p = x NPB + g, + gridDim.x NPB, ... (NPB = CT_THREADS / K per block and
+// pass); lane k of the group is cell k of the tile.  The record load is one
+// address per group; the f loads and the value stores are K contiguous
+// elements (a 64- or 128-byte row).
+// Dv holds the chunk's D values [j][p][K].
+template <class T, int K>
+__global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const uint4 *__restrict__ nrec, double Kw,
+                                                        const T *__restrict__ ft, int t0, T *__restrict__ Dv,
+                                                        const double *__restrict__ kmt)
+{
+    __shared__ double ts[KMT_SIZE];
+    if (sizeof(T) == 8)
+        for (int i = threadIdx.x; i < KMT_SIZE; i += CT_THREADS) ts[i] = kmt[i];
+    __syncthreads();
+    constexpr int NPB = CT_THREADS / K;
+    const int k = threadIdx.x % K, g = ct_group<K>();
+    const T *fr = ft + (size_t)(t0 + blockIdx.y) * npts * K;
+    T *Dt = Dv + (size_t)blockIdx.y * nn * K;
+    const T kw = (T)Kw;
+    auto ld = [&](uint32_t i) { return ct_row<K>(fr, i, k); };
+    // two nodes per pass (p and p + stride), evaluated together so that their
+    // loads are in flight at once; plain stores: the gather reads the values
+    // back soon, and streaming stores would keep them out of the caches on
+    // the way to HBM
+    auto store = [&](int p, T d) {
+        if (K == WAVE)
+            (Dt + (size_t)p * K)[k] = d;
+        else
+            *(T *)((char *)Dt + (size_t)(uint32_t)(((uint32_t)p * K + k) * (uint32_t)sizeof(T))) = d;
+    };
+    const int stride = gridDim.x * NPB;
+    for (int p = blockIdx.x * NPB + g; p < nn; p += 2 * stride) {
+        const int p2 = p + stride < nn ? p + stride : p;  // clamped: in bounds
+        const uint4 x = nrec[p], x2 = nrec[p2];
+        const T d = ct_terms(x, ld, ts, kw);
+        const T d2 = ct_terms(x2, ld, ts, kw);
+        store(p, d);
+        if (p + stride < nn) store(p2, d2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gather
+// ---------------------------------------------------------------------------
+// Lane group g of block (x, j) takes destination slots s = x NPB + g, +
+// gridDim.x NPB, ... of the longest-first order dperm; lane k sums cell k of
+// the destination in the CSR's order: q = sum coef D (coef -1 for role 0) and
+// l = sum 2 w f_partner over the role-0 entries, then L = f_d l.  FP64
+// products and sums; the FP64 tiles keep the single-cell path's term for q
+// (coef D, then the sum), the mixed form one FMA per term.  Each round issues
+// the loads of CT_UNROLL entries (records, then D values and f values) before
+// adding any of them, with no branch between the loads: every entry loads a D
+// value and an f value (its partner's for role 0, the destination's own --
+// one cached row -- otherwise); entries past the destination's end are
+// clamped to its last one and add exact zeros (+-0 leaves a sum unchanged).
+template <class T, int K>
+__global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, const int32_t *__restrict__ cptr,
+                                                          const uint4 *__restrict__ cent,
+                                                          const int32_t *__restrict__ dperm,
+                                                          const T *__restrict__ Dv, const T *__restrict__ ft, int t0,
+                                                          double *__restrict__ Q0, double *__restrict__ L)
+{
+#pragma clang fp contract(off)
+    constexpr int NPB = CT_THREADS / K;
+    const int k = threadIdx.x % K, g = ct_group<K>();
+    const T *Dt = Dv + (size_t)blockIdx.y * nn * K;
+    const T *fr = ft + (size_t)(t0 + blockIdx.y) * npts * K;
+    const size_t to = (size_t)(t0 + blockIdx.y) * npts * K;
+    for (int s = blockIdx.x * NPB + g; s < npts; s += gridDim.x * NPB) {
+        const int d = dperm[s];
+        const int e0 = cptr[d], e1 = cptr[d + 1];
+        double q = 0.0, l = 0.0;
+        for (int e = e0; e < e1; e += CT_UNROLL) {
+            uint4 x[CT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < CT_UNROLL; u++) {  // clamped to the destination's last entry: in bounds
+                const uint32_t eu = (uint32_t)min(e + u, e1 - 1);
+                x[u] = K == WAVE ? cent[eu] : ld_off(cent, eu);
+            }
+            T dv[CT_UNROLL], fv[CT_UNROLL];
+#pragma unroll
+            for (int u = 0; u < CT_UNROLL; u++) {
+                dv[u] = ct_row<K>(Dt, x[u].x >> 2, k);
+                // a wave per destination: the role is uniform, so only role-0
+                // entries load their partner's f (no divergence)
+                // FP64 rows with a wave per destination: the role is uniform,
+                // so only role-0 entries load their partner's f (3% faster);
+                // FP32: every entry loads (the branches cost more than the
+                // cached loads: 1.85 against 1.62 us per cell)
+                if (K == WAVE && sizeof(T) == 8)
+                    fv[u] = (x[u].x & 3u) == 0u ? ct_row<K>(fr, x[u].y, k) : (T)0;
+                else
+                    fv[u] = ct_row<K>(fr, x[u].y, k);
+            }
+#pragma unroll
+            for (int u = 0; u < CT_UNROLL; u++) {
+                const bool in = e + u < e1, role0 = (x[u].x & 3u) == 0u;
+                const double val = __hiloint2double((int)x[u].w, (int)x[u].z);
+                const double coef = in ? (role0 ? -1.0 : val) : 0.0;
+                if (sizeof(T) == 8)
+                    q += coef * (double)dv[u];  // the single-cell path's term, then the sum
+                else
+                    q = fma(coef, (double)dv[u], q);  // mixed: one rounding per term
+                l += (in && role0) ? val * (double)fv[u] : 0.0;
+            }
+        }
+        Q0[to + (size_t)d * K + k] = q;
+        L[to + (size_t)d * K + k] = (double)ct_row<K>(fr, (uint32_t)d, k) * l;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// moments, solve, apply
+// ---------------------------------------------------------------------------
+// Block (b, t): destinations [b npts / P, (b + 1) npts / P) of tile t; lane
+// group g of K lanes takes destinations lo + g, + NPB, ...; lane k adds cell
+// k's moments.  The lanes of one cell are reduced across the wave (lane bits
+// above K) and then across the block's waves in a fixed order; partial b of
+// cell (t K + k) goes to mp[((t K + k) 20 + m) P + b].
+template <int K>
+__global__ __launch_bounds__(CT_THREADS) void k_ct_moments(int npts, const double *__restrict__ xi,
+                                                           const double *__restrict__ Q0,
+                                                           const double *__restrict__ L, double *__restrict__ mp)
+{
+    constexpr int NPB = CT_THREADS / K, NW = CT_THREADS / WAVE;
+    __shared__ double red[NW][K][21];
+    const int k = threadIdx.x % K, g = threadIdx.x / K, t = blockIdx.y, P = gridDim.x;
+    const int lo = (int)((long)blockIdx.x * npts / P), hi = (int)((long)(blockIdx.x + 1) * npts / P);
+    const size_t to = (size_t)t * npts * K;
+    double m[20];
+#pragma unroll
+    for (int i = 0; i < 20; i++) m[i] = 0.0;
+    for (int d = lo + g; d < hi; d += NPB)
+        moments_add(m, xi, d, Q0[to + (size_t)d * K + k], L[to + (size_t)d * K + k]);
+#pragma unroll
+    for (int sh = K; sh < WAVE; sh <<= 1)
+#pragma unroll
+        for (int i = 0; i < 20; i++) m[i] += __shfl_xor(m[i], sh, WAVE);
+    const int wv = threadIdx.x / WAVE;
+    if (threadIdx.x % WAVE < K)
+#pragma unroll
+        for (int i = 0; i < 20; i++) red[wv][k][i] = m[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < K * 20; i += CT_THREADS) {
+        const int kk = i / 20, mm = i % 20;
+        double sm = 0.0;
+        for (int w = 0; w < NW; w++) sm += red[w][kk][mm];
+        mp[(((size_t)t * K + kk) * 20 + mm) * P + blockIdx.x] = sm;
+    }
+}
+
+// One wave per cell: lane i adds partials i, i + 64, ... of each moment, the
+// wave sums them (fixed pairing), then the 5x5 solve; X[c][5].
+__global__ __launch_bounds__(WAVE) void k_ct_solve(int cells, int P, const double *__restrict__ mp,
+                                                   double *__restrict__ X)
+{
+    const int c = blockIdx.x;
+    if (c >= cells) return;
+    double s[20];
+#pragma unroll
+    for (int i = 0; i < 20; i++) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < P; b += WAVE) a += mp[((size_t)c * 20 + i) * P + b];
+        s[i] = a;
+    }
+    wave_sum_n<20>(s);
+    double x[5];
+    moments_solve(s, x);
+    if (threadIdx.x < 5) {
+        double v = x[0];
+#pragma unroll
+        for (int i = 1; i < 5; i++) v = threadIdx.x == i ? x[i] : v;
+        X[(size_t)c * 5 + threadIdx.x] = v;
+    }
+}
+
+// Destinations per apply block: its LDS holds two [K][DCH + 1] FP64 tiles.
+__host__ __device__ constexpr int ct_apply_dests(int K) { return K <= 32 ? 64 : 32; }
+
+// Block (x, t): destinations [DCH x, DCH x + DCH) of tile t.  Q0 / L rows go
+// through LDS so that every output row (a cell's 64 destinations) is written
+// contiguously (DCH destinations):  Q = Q0 - L s, s = X.psi (X = 0 uncorrected); loss = L;
+// nu = (L / f)(1 + s) (0 where f <= 0); f += dt Q.
+template <int K>
+__global__ __launch_bounds__(CT_THREADS) void k_ct_apply(int npts, int cells, const double *__restrict__ xi,
+                                                         const double *__restrict__ Q0,
+                                                         const double *__restrict__ L,
+                                                         const double *__restrict__ X, const double *f,
+                                                         double *__restrict__ Qout, double *__restrict__ Lout,
+                                                         double *__restrict__ nu_out, double dt, double *f_upd)
+{
+    constexpr int DCH = ct_apply_dests(K);
+    __shared__ double sq[K][DCH + 1], sl[K][DCH + 1];
+    const int d0 = blockIdx.x * DCH, t = blockIdx.y;
+    const size_t to = (size_t)t * npts * K;
+    for (int i = threadIdx.x; i < DCH * K; i += CT_THREADS) {
+        const int dl = i / K, k = i % K, d = d0 + dl;
+        if (d < npts) {
+            sq[k][dl] = Q0[to + (size_t)d * K + k];
+            sl[k][dl] = L[to + (size_t)d * K + k];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < DCH * K; i += CT_THREADS) {
+        const int k = i / DCH, dl = i % DCH, d = d0 + dl;
+        const long c = (long)t * K + k;
+        if (c >= cells || d >= npts) continue;
+        const double x = xi[3 * d], y = xi[3 * d + 1], z = xi[3 * d + 2];
+        const double e = x * x + y * y + z * z;
+        double s = 0.0;
+        if (X) {
+            const double *Xc = X + 5 * c;
+            s = Xc[0] + Xc[1] * x + Xc[2] * y + Xc[3] * z + Xc[4] * e;
+        }
+        const double lv = sl[k][dl], q = sq[k][dl] - lv * s;
+        const size_t o = (size_t)c * npts + d;
+        if (Qout) Qout[o] = q;
+        if (Lout) Lout[o] = lv;
+        if (nu_out || dt != 0.0) {
+            const double fv = f[o];
+            if (nu_out) nu_out[o] = fv > 0.0 ? (lv / fv) * (1.0 + s) : 0.0;
+            if (dt != 0.0) f_upd[o] = fv + dt * q;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int ct_cells_per_tile(const gdt_context *h, bool mixed)
+{
+    return h->ct_wide ? WAVE : (mixed ? CtK<float>::K : CtK<double>::K);
+}
+
+int ct_prepare(gdt_context *h, int cells, bool mixed)
+{
+    const int K = ct_cells_per_tile(h, mixed);
+    const size_t sz = mixed ? sizeof(float) : sizeof(double);
+    const long tiles = (cells + K - 1) / K;
+    const long chunk = std::min<long>(tiles, h->ct_chunk_tiles);
+    const size_t np = h->n_pnt;
+    // node values of a chunk and an f tile are indexed with 32-bit byte offsets
+    GDT_CHECK((double)sz * chunk * h->nn * K < 4294967296.0 && h->nn < (1L << 30), GDT_E_ARG,
+              "cell tiles: %ld node values per chunk exceed the 32-bit offsets", chunk * h->nn * K);
+    GDT_CHECK(h->ct_f.ensure(sz * tiles * np * K) == 0 && h->ct_D.ensure(sz * chunk * h->nn * K) == 0 &&
+                  h->ct_Q0.ensure(8 * tiles * np * K) == 0 &&
+                  h->ct_L.ensure(8 * tiles * np * K) == 0 &&
+                  h->ct_mp.ensure(8 * (size_t)tiles * K * 20 * CT_MOM_PARTS) == 0 &&
+                  h->ct_X.ensure(8 * (size_t)tiles * K * 5) == 0,
+              GDT_E_NOMEM, "cell tiles: scratch allocation failed (%ld tiles of %d cells)", tiles, K);
+    return GDT_OK;
+}
+
+template <class T, int K>
+static int ct_launch(gdt_context *h, const double *f, int cells, bool correct, double *Qout, double *Lout,
+                     double *nu_out, double dt, double *f_upd)
+{
+    const int np = h->n_pnt, tiles = (cells + K - 1) / K;
+    const int chunk = std::max(1, std::min(tiles, h->ct_chunk_tiles));
+    T *ft = h->ct_f.as<T>(), *Dv = h->ct_D.as<T>();
+    double *Q0 = h->ct_Q0.as<double>(), *L = h->ct_L.as<double>();
+    GDT_LAUNCH_TIMED(h, "ct_pack", (k_ct_pack<T, K>), dim3((np + 63) / 64, tiles), dim3(CT_THREADS), 0u, np, cells,
+                     f, ft);
+    const int nblk = std::max(1, h->ct_node_blocks), gblk = std::max(1, h->ct_gather_blocks);
+    for (int t0 = 0; t0 < tiles; t0 += chunk) {
+        const int nt = std::min(chunk, tiles - t0);
+        GDT_LAUNCH_TIMED(h, "ct_node", (k_ct_node<T, K>), dim3(nblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
+                         h->kn4.as<uint4>(), h->kK, (const T *)ft, t0, Dv, h->kmath.as<double>());
+        GDT_LAUNCH_TIMED(h, "ct_gather", (k_ct_gather<T, K>), dim3(gblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
+                         h->kcptr.as<int32_t>(), h->kcent.as<uint4>(), h->kdperm.as<int32_t>(), (const T *)Dv,
+                         (const T *)ft, t0, Q0, L);
+    }
+    double *X = nullptr;
+    if (correct) {
+        GDT_LAUNCH_TIMED(h, "ct_moments", k_ct_moments<K>, dim3(CT_MOM_PARTS, tiles), dim3(CT_THREADS), 0u, np,
+                         h->xi.as<double>(), (const double *)Q0, (const double *)L, h->ct_mp.as<double>());
+        GDT_LAUNCH_TIMED(h, "ct_solve", k_ct_solve, dim3(cells), dim3(WAVE), 0u, cells, CT_MOM_PARTS,
+                         (const double *)h->ct_mp.as<double>(), h->ct_X.as<double>());
+        X = h->ct_X.as<double>();
+    }
+    GDT_LAUNCH_TIMED(h, "ct_apply", k_ct_apply<K>, dim3((np + ct_apply_dests(K) - 1) / ct_apply_dests(K), tiles), dim3(CT_THREADS), 0u, np, cells,
+                     h->xi.as<double>(), (const double *)Q0, (const double *)L, (const double *)X, f, Qout, Lout,
+                     nu_out, dt, f_upd);
+    GDT_HIP(hipGetLastError());
+    return GDT_OK;
+}
+
+int ct_config(gdt_context *h, int chunk_tiles, int node_blocks, int gather_blocks, int wide)
+{
+    GDT_CHECK(chunk_tiles >= 1 && chunk_tiles <= 64 && node_blocks >= 1 && node_blocks <= 65535 &&
+                  gather_blocks >= 1 && gather_blocks <= 65535 && (wide == 0 || wide == 1),
+              GDT_E_ARG, "cell tiles: launch shape (%d, %d, %d, %d) out of range", chunk_tiles, node_blocks,
+              gather_blocks, wide);
+    h->ct_chunk_tiles = chunk_tiles;
+    h->ct_node_blocks = node_blocks;
+    h->ct_gather_blocks = gather_blocks;
+    h->ct_wide = wide;
+    return GDT_OK;
+}
+
+int ct_run(gdt_context *h, bool mixed, const double *f, int cells, bool correct, double *Qout, double *Lout,
+           double *nu_out, double dt, double *f_upd)
+{
+    GDT_CHECK(h->kcent.p && h->kcptr.p && h->kdperm.p, GDT_E_STATE, "cell tiles: stencil not built");
+    if (h->ct_wide)
+        return mixed ? ct_launch<float, WAVE>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd)
+                     : ct_launch<double, WAVE>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd);
+    return mixed ? ct_launch<float, CtK<float>::K>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd)
+                 : ct_launch<double, CtK<double>::K>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd);
+}

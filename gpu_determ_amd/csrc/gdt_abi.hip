@@ -689,6 +689,55 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 gent[2 * (size_t)pos + 1] = n4[4 * (size_t)p + 3];
             }
         }
+        // one destination-sorted CSR over the sorted positions (the
+        // cell-tile gather): a destination's entries follow (p, slot), i.e.
+        // its group CSRs' entries in group order
+        {
+            std::vector<int32_t> cp(np + 1, 0);
+            for (long p = 0; p < nn; p++) {
+                const long t = order[p];
+                for (int sl = 0; sl < slots(t); sl++) cp[cidx6[6 * t + sl] + 1]++;
+            }
+            for (int v = 0; v < np; v++) cp[v + 1] += cp[v];
+            // entry record (p << 2 | role, g, value as FP64 bits):
+            //   role 0 (alpha, beta):  g = the partner node (beta, alpha),
+            //                          value = 2 w (the loss is f_d sum 2 w f_g);
+            //   role 1 / 2:            g = the destination itself, value =
+            //                          1 - r / r, r = (num / 4) / (den / 4) --
+            //                          the quotient the group gather forms
+            std::vector<uint32_t> cent(4 * (size_t)ne);
+            std::vector<int32_t> cf(cp.begin(), cp.end() - 1);
+            for (long p = 0; p < nn; p++) {
+                const long t = order[p];
+                const uint32_t y = n4[4 * (size_t)p + 3];
+                const double num4 = (double)(y & 0x7FFu), den4 = (double)((y >> 11) & 0x7FFu);
+                const double r = num4 == 0.0 ? 0.0 : num4 / den4;
+                for (int sl = 0; sl < slots(t); sl++) {
+                    const uint32_t role = sl < 2 ? 0u : (sl < 4 ? 1u : 2u);
+                    const int32_t dst = cidx6[6 * t + sl];
+                    const double val = role == 0u ? 2.0 * cw[t] : (role == 1u ? 1.0 - r : r);
+                    const int32_t g = role == 0u ? cidx6[6 * t + (1 - sl)] : dst;
+                    uint64_t cb;
+                    std::memcpy(&cb, &val, sizeof cb);
+                    const int32_t pos = cf[dst]++;
+                    cent[4 * (size_t)pos] = ((uint32_t)p << 2) | role;
+                    cent[4 * (size_t)pos + 1] = (uint32_t)g;
+                    cent[4 * (size_t)pos + 2] = (uint32_t)cb;
+                    cent[4 * (size_t)pos + 3] = (uint32_t)(cb >> 32);
+                }
+            }
+            std::vector<int32_t> dperm(np);
+            for (int v = 0; v < np; v++) dperm[v] = v;
+            std::stable_sort(dperm.begin(), dperm.end(),
+                             [&](int32_t x, int32_t y) { return cp[x + 1] - cp[x] > cp[y + 1] - cp[y]; });
+            GDT_CHECK(h->kcptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
+                          h->kcent.ensure(sizeof(uint32_t) * 4 * ne) == 0 &&
+                          h->kdperm.ensure(sizeof(int32_t) * np) == 0,
+                      GDT_E_NOMEM, "gdt_korobov_build: cell-tile CSR allocation failed");
+            GDT_HIP(hipMemcpy(h->kcptr.p, cp.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->kcent.p, cent.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->kdperm.p, dperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+        }
         h->h_gp.swap(gp);
         GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * h->h_gp.size()) == 0 &&
                       h->kgent.ensure(sizeof(uint32_t) * 2 * ne) == 0,
@@ -784,6 +833,7 @@ static int collide_matrix_cm(gdt_context *h, const double *f_cm, double *nu, dou
 }
 
 static bool use_fused(unsigned flags) { return (flags & GDT_FLAG_FUSED) != 0; }
+static bool use_tiles(unsigned flags) { return (flags & (GDT_FLAG_CELL_TILES | GDT_FLAG_MIXED)) != 0; }
 
 // Scratch of the Korobov path for `cells` cells: moment sums, node values,
 // group partials, and the fused path's wave partition (outside any capture).
@@ -911,12 +961,21 @@ int gdt_evaluate(gdt_handle h, int method, const double *f, double *nu, double *
         } else if (method == GDT_METHOD_KOROBOV) {
             GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_evaluate: stencil not built");
             double *L = h->s_loss.as<double>();
-            KorPartials kp;
-            if ((rc = kor_partials(h, fcm, cells, flags, kp))) goto done;
-            // q_c = corrected Q, nu_c = corrected loss frequency, L = loss
-            if ((rc = launch_kor_apply(h, (flags & GDT_FLAG_CORRECT) != 0, kp.G, kp.nparts, h->s_mpart.as<double>(),
-                                       fcm, kp.L, kp.Q, q_c, L, nu_c, nullptr, cells, 0.0, nullptr)))
-                goto done;
+            if (use_tiles(flags)) {
+                const bool mixed = (flags & GDT_FLAG_MIXED) != 0;
+                if ((rc = ct_prepare(h, cells, mixed))) goto done;
+                if ((rc = ct_run(h, mixed, fcm, cells, (flags & GDT_FLAG_CORRECT) != 0, q_c, L, nu_c, 0.0,
+                                 nullptr)))
+                    goto done;
+            } else {
+                KorPartials kp;
+                if ((rc = kor_partials(h, fcm, cells, flags, kp))) goto done;
+                // q_c = corrected Q, nu_c = corrected loss frequency, L = loss
+                if ((rc = launch_kor_apply(h, (flags & GDT_FLAG_CORRECT) != 0, kp.G, kp.nparts,
+                                           h->s_mpart.as<double>(), fcm, kp.L, kp.Q, q_c, L, nu_c, nullptr, cells,
+                                           0.0, nullptr)))
+                    goto done;
+            }
             // gain = Q + f nu_c
             if ((rc = launch_correct(h, -3, fcm, nu_c, q_c, nullptr, g_c, nullptr, cells, 0.0, nullptr))) goto done;
         } else {
@@ -977,6 +1036,8 @@ static int one_step(gdt_context *h, int method, double *f, int cells, double dt,
 {
     int rc;
     const bool corr = (flags & GDT_FLAG_CORRECT) != 0;
+    if (method == GDT_METHOD_KOROBOV && use_tiles(flags))
+        return ct_run(h, (flags & GDT_FLAG_MIXED) != 0, f, cells, corr, nullptr, nullptr, nullptr, dt, f);
     if (method == GDT_METHOD_KOROBOV) {
         KorPartials kp;
         if ((rc = kor_partials(h, f, cells, flags, kp))) return rc;
@@ -1004,7 +1065,7 @@ int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int
         GDT_CHECK(h->s_fbox.ensure(sizeof(double) * nbox) == 0 && h->s_out1.ensure(sizeof(double) * nbox) == 0,
                   GDT_E_NOMEM, "gdt_run_steps: scratch allocation failed");
     } else {
-        int rc0 = kor_prepare(h, cells);
+        int rc0 = use_tiles(flags) ? ct_prepare(h, cells, (flags & GDT_FLAG_MIXED) != 0) : kor_prepare(h, cells);
         if (rc0) return rc0;
     }
     const bool graph = (flags & GDT_FLAG_GRAPH) && !(flags & GDT_FLAG_TIME);
@@ -1026,7 +1087,7 @@ int gdt_run_steps(gdt_handle h, int method, double *f, int cells, double dt, int
             break;
         }
     if (chunk < 8 && steps > 64) chunk = 64;
-    const unsigned key = flags & (GDT_FLAG_CORRECT | GDT_FLAG_FUSED);
+    const unsigned key = flags & (GDT_FLAG_CORRECT | GDT_FLAG_FUSED | GDT_FLAG_CELL_TILES | GDT_FLAG_MIXED);
     // the captured kernels hold raw pointers to scratch and table buffers:
     // any DevBuf freed since the capture (a larger call elsewhere, a table
     // rebuild) makes the graph stale
@@ -1125,6 +1186,18 @@ int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap)
     if (tiles) *tiles = h->kntile;
     if (cap) *cap = h->kcap;
     return GDT_OK;
+}
+
+int gdt_korobov_cell_tiles(gdt_handle h, int chunk_tiles, int node_blocks, int gather_blocks, int wide)
+{
+    H_CHECK(h);
+    GDT_HIP(hipSetDevice(h->device));
+    GDT_HIP(hipStreamSynchronize(h->stream));  // no evaluation in flight uses the old shape
+    if (h->graph_exec) {                        // captured launches carry the old grids
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+    }
+    return ct_config(h, chunk_tiles, node_blocks, gather_blocks, wide);
 }
 
 int gdt_korobov_set_tiles(gdt_handle h, int tiles_max)

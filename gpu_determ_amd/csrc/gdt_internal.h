@@ -94,6 +94,10 @@ struct gdt_context {
     std::vector<int32_t> h_gp;    // host copy of the group CSR row pointers [G][n+1]
     unsigned kor_gen = 0;         // bumped whenever the stencil's gather layout changes
     std::vector<int32_t> h_kptr;  // host copy of the CSR row pointers
+    // many-cell (cell-tile) path: one destination-sorted CSR over the sorted
+    // node positions (uint4 entry records: p << 2 | role, a grid node, an FP64
+    // value), and the destinations ordered by entry count, longest first
+    DevBuf kcptr, kcent, kdperm;
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
     int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
     int gain_lds_attr = 0;        // dynamic-LDS bytes the matrix gain kernel is enabled for
@@ -105,6 +109,13 @@ struct gdt_context {
 
     // scratch
     DevBuf s_fbox, s_out1, s_out2, s_vals, s_loss, s_Q, s_nu, s_tmp, s_dt, s_mpart, s_part;
+    // cell-tile scratch: f tiles, node values D of a chunk of tiles, per-tile
+    // Q0 / L rows, moment partials, correction coefficients
+    DevBuf ct_f, ct_D, ct_Q0, ct_L, ct_mp, ct_X;
+    int ct_chunk_tiles = 4;      // tiles whose node values are in flight between node and gather
+    int ct_node_blocks = 2048;   // grid of the node kernel per tile (grid-stride over nodes)
+    int ct_gather_blocks = 4096; // grid of the gather per tile (grid-stride over destinations)
+    int ct_wide = 1;             // 1: tiles of 64 cells (a wave per node / destination)
 
     // step graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -250,6 +261,16 @@ __host__ __device__ constexpr int kor_fpad(int npts) { return (npts + 127) & ~12
 // tables are at most GLIST_AXIS_MAX.
 #define GLIST_AXIS_MAX 1024
 __host__ __device__ constexpr int glist_pack(int in, int jn, int kn) { return in | jn << 10 | kn << 20; }
+// ---- many-cell path in cell tiles (cell_tiles.hip) ----------------------
+int ct_cells_per_tile(const gdt_context *h, bool mixed);
+// Scratch for `cells` cells (outside any stream capture).
+int ct_prepare(gdt_context *h, int cells, bool mixed);
+// Korobov evaluation of `cells` cell-major cells: FP64 node terms, or FP32
+// (mixed) with FP64 sums; outputs cell-major (any may be NULL), corrected when
+// `correct`; with dt != 0, f_upd <- f + dt Q.
+int ct_config(gdt_context *h, int chunk_tiles, int node_blocks, int gather_blocks, int wide);
+int ct_run(gdt_context *h, bool mixed, const double *f, int cells, bool correct, double *Qout, double *Lout,
+           double *nu_out, double dt, double *f_upd);
 int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells);
 // G group partials [cell][G][node] of Q and loss (G = 1: plain arrays)
 int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const double *mpart, const double *f,

@@ -50,6 +50,13 @@ extern "C" {
 #define GDT_FLAG_FUSED 16u     /* Korobov: one fused kernel recomputing each
                                   node's term per CSR entry, instead of the
                                   node-value + gather pair                  */
+#define GDT_FLAG_CELL_TILES 32u /* Korobov, many cells: cells interleaved in
+                                  tiles of 16 that share every stencil read
+                                  (FP64 node terms and sums)                */
+#define GDT_FLAG_MIXED 64u     /* Korobov, many cells: tiles of 32 cells with
+                                  f and the node terms in FP32, every sum and
+                                  the correction in FP64 (relative L2 error
+                                  of Q ~1e-6 .. 2e-5 against FP64)          */
 
 typedef struct gdt_context *gdt_handle;
 
@@ -138,6 +145,15 @@ int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
  * batch holding this context must be configured again (gdt_batch_config);
  * gdt_batch_tune sets it on the batch's classes. */
 int gdt_korobov_set_tiles(gdt_handle h, int tiles_max);
+/* Launch shape of the cell-tile paths (GDT_FLAG_CELL_TILES / GDT_FLAG_MIXED),
+ * no reference counterpart: tiles whose node values are in flight between
+ * the node and gather kernels (1..64), grid sizes of the node and gather
+ * kernels per tile, and the tile width (wide = 0: 16 cells in FP64, 32 in
+ * FP32; wide = 1: 64 cells, a wave per node and per destination).  A
+ * destination's sums follow the CSR order whatever the shape, so results are
+ * the same bits for every chunk and grid size; the tile width changes the
+ * order of the moment sums, so the corrected Q in its last bits. */
+int gdt_korobov_cell_tiles(gdt_handle h, int chunk_tiles, int node_blocks, int gather_blocks, int wide);
 /* Accuracy hook, no reference counterpart: out[i] = x[i]^r[i] as the node
  * terms compute it (table-driven FP64 log and exp); device pointers, x > 0;
  * synchronous. */
