@@ -49,7 +49,7 @@ ABI_SYMBOLS = (
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
     "gdt_init_mixture", "gdt_korobov_tiles", "gdt_korobov_set_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
-    "gdt_transport", "gdt_korobov_cell_tiles",
+    "gdt_transport", "gdt_korobov_cell_tiles", "gdt_evaluate_noalloc",
 )
 
 _lib = None
@@ -102,6 +102,7 @@ def lib():
         "gdt_collide_korobov": [vp, vp, vp, vp, C.c_int, C.c_int],
         "gdt_correct": [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int],
         "gdt_evaluate": [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_uint],
+        "gdt_evaluate_noalloc": [vp, vp, vp, vp, vp, C.c_int, C.c_uint],
         "gdt_relax": [vp, vp, vp, vp, C.c_int, C.c_double, C.c_double, vp, C.c_uint],
         "gdt_device_synchronize": [],
         "gdt_convert_f32_to_f64": [vp, vp, C.c_size_t],
@@ -339,6 +340,15 @@ class Context:
         Q = DeviceArray((cells, self.n_pnt))
         check(lib().gdt_evaluate(self.h, int(method), f.ptr, nu.ptr, gain.ptr, Q.ptr, int(cells), int(layout),
                                  int(flags)))
+        return nu, gain, Q
+
+    def evaluate_noalloc(self, f, cells, flags=FLAG_CORRECT, nu=None, gain=None, Q=None):
+        """gdt_evaluate_noalloc (Korobov, cell-major): outputs into the given
+        DeviceArrays (new ones when None); the library allocates nothing."""
+        nu = DeviceArray((cells, self.n_pnt)) if nu is None else nu
+        gain = DeviceArray((cells, self.n_pnt)) if gain is None else gain
+        Q = DeviceArray((cells, self.n_pnt)) if Q is None else Q
+        check(lib().gdt_evaluate_noalloc(self.h, f.ptr, nu.ptr, gain.ptr, Q.ptr, int(cells), int(flags)))
         return nu, gain, Q
 
     def correct(self, method, f, nu, gain, Q, cells, X=None):

@@ -28,7 +28,7 @@ HIP_FLAGS = [
 ]
 # Setup kernels select integers from floating-point comparisons: no contraction.
 EXTRA = {"setup_kernels.hip": ["-ffp-contract=off"]}
-HIP_SOURCES = ["setup_kernels.hip", "collide_kernels.hip", "korobov_kernels.hip", "gdt_abi.hip", "batch_abi.hip", "mfma_gemm.hip", "matrix_mfma.hip", "mfma_abi.hip", "surrogate.hip", "transport.hip", "cell_tiles.hip"]
+HIP_SOURCES = ["setup_kernels.hip", "collide_kernels.hip", "korobov_kernels.hip", "gdt_abi.hip", "batch_abi.hip", "mfma_gemm.hip", "matrix_mfma.hip", "mfma_abi.hip", "surrogate.hip", "transport.hip", "cell_tiles.hip", "noalloc.hip"]
 COMPAT_SOURCES = ["options.cpp", "velocity_grid.cpp", "compat.cpp"]
 
 

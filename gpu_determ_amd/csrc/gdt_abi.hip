@@ -238,6 +238,10 @@ int gdt_create(gdt_handle *out, const gdt_grid_desc *gd, int device)
     }
     GDT_CHECK(device >= 0 && device < ndev, GDT_E_ARG, "gdt_create: device %d of %d", device, ndev);
     GDT_HIP(hipSetDevice(device));
+    {
+        const int rc0 = noalloc_setup(device);  // scratch address of gdt_evaluate_noalloc on this device
+        if (rc0 != GDT_OK) return rc0;
+    }
 
     gdt_context *h = new gdt_context();
     h->device = device;

@@ -261,6 +261,11 @@ __host__ __device__ constexpr int kor_fpad(int npts) { return (npts + 127) & ~12
 // tables are at most GLIST_AXIS_MAX.
 #define GLIST_AXIS_MAX 1024
 __host__ __device__ constexpr int glist_pack(int in, int jn, int kn) { return in | jn << 10 | kn << 20; }
+// ---- allocation-free evaluation (noalloc.hip) --------------------------
+// Device address of the static scratch array on `device` (current), looked
+// up once per device.
+int noalloc_setup(int device);
+
 // ---- many-cell path in cell tiles (cell_tiles.hip) ----------------------
 int ct_cells_per_tile(const gdt_context *h, bool mixed);
 // Scratch for `cells` cells (outside any stream capture).

@@ -188,6 +188,19 @@ int gdt_correct(gdt_handle h, int method, const double *f, double *nu, const dou
  * loss frequency; KOROBOV: loss/f), gain (KOROBOV: Q + loss), Q. */
 int gdt_evaluate(gdt_handle h, int method, const double *f, double *nu, double *gain, double *Q, int cells,
                  int layout, unsigned flags);
+/* Allocation-free variant of gdt_evaluate for the Korobov method, with no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and results as gdt_evaluate(h,
+ * GDT_METHOD_KOROBOV, f, nu, gain, Q, cells, GDT_LAYOUT_CELL_MAJOR, flags),
+ * bit for bit; flags is 0 or GDT_FLAG_CORRECT; nu and gain may be NULL (gain
+ * needs nu).  Its scratch (node values, group partials, moment sums) is one
+ * static 64 MiB device array per GPU: calls needing more (more than about 7
+ * cells of a 32^3 grid with 2^21 lattice nodes) return GDT_E_ARG.  Every
+ * call on a GPU shares that array, so calls that overlap on one GPU race.
+ * Calls no hipMalloc or hipFree; asynchronous on the handle's stream. */
+int gdt_evaluate_noalloc(gdt_handle h, const double *f, double *nu, double *gain, double *Q, int cells,
+                         unsigned flags);
 /* relax_f (collision_integral.cu:835-880): f <- f (1 - nu dt) + gain dt for
  * all cells, with dt = min courant / nu over every value (flags &
  * GDT_FLAG_COURANT, get_delta_t :776-807) or the given dt.  time (device
