@@ -303,6 +303,112 @@ def surrogate_form(G, train_cells=200000, infer_cells=16384, epochs=10, threshol
     return out
 
 
+def mixture_params(rng, cells, umax=1.2):
+    """(n, T, u) of 1-3 Maxwellian components per cell: n ~ U(0.2,1), T ~
+    U(0.3,1.5), u uniform in the ball |u| <= umax; absent components have n = 0."""
+    comps = rng.integers(1, 4, cells)
+    par = np.zeros((cells, 3, 5))
+    for j in range(3):
+        u = rng.uniform(-1.0, 1.0, (cells, 3))
+        u *= (umax * rng.uniform(0, 1, cells) ** (1 / 3) / np.maximum(np.linalg.norm(u, axis=1), 1e-12))[:, None]
+        par[:, j, 0] = rng.uniform(0.2, 1.0, cells) * (comps > j)
+        par[:, j, 1] = rng.uniform(0.3, 1.5, cells)
+        par[:, j, 2:] = u
+    return par
+
+
+def conservation_residual(xi, Q, d3v):
+    """max over cells of |sum Q psi| / sum |Q| |psi| (psi = 1, xi, |xi|^2)."""
+    e = np.sum(xi * xi, axis=1)
+    psi = np.stack([np.ones(len(e)), xi[:, 0], xi[:, 1], xi[:, 2], e])
+    m = np.abs(Q @ psi.T) * d3v
+    sc = (np.abs(Q) @ np.abs(psi.T)) * d3v
+    return float(np.max(m / np.maximum(sc, 1e-300)))
+
+
+def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed=SEED):
+    """The metric's comparison on one set of cells at 32^3 (vmax 4.8, 17,256
+    nodes, 2^21 lattice nodes, s0 = 0.25): `cells` seeded mixtures of 1-3
+    Maxwellians.  Throughput unit: one corrected Q(f,f) evaluation with the
+    Euler update (dt = 0.02) of one cell, as the headline.  Forms:
+      plain_fp64_single   the headline (one cell per step, the grouped kernels)
+      plain_fp64_grouped  the same kernels on all cells at once
+      plain_fp64_tiles    FP64 cell tiles (64 cells per row, same node terms)
+      mixed_fp32_tiles    cell tiles with f and the node terms in FP32, sums
+                          and the 5x5 correction in FP64
+    Accuracy on the unstepped cells: corrected Q against the FP64 tiles over
+    every cell (the FP64 tiles are within 1e-12 of the oracle; checked here on
+    `oracle_cells` cells too), the worst cell, and the conservation residual
+    of every form."""
+    rng = np.random.default_rng(seed + 7)
+    n = ctx.n_pnt
+    f0 = ctx.init_mixture(mixture_params(rng, cells))
+    fh = f0.get()
+    forms = {"plain_fp64_grouped": G.FLAG_CORRECT, "plain_fp64_tiles": G.FLAG_CORRECT | G.FLAG_CELL_TILES,
+             "mixed_fp32_tiles": G.FLAG_CORRECT | G.FLAG_MIXED}
+    out = {"workload": "%d cells, 32^3 sphere cut (17,256 nodes), 2^21 lattice nodes, seeded 1-3 Maxwellian "
+                       "mixtures; unit = one corrected evaluation + Euler step of one cell" % cells,
+           "cells": cells, "steps_timed": steps,
+           "plain_fp64_single": {"evals_per_s": round(single_evals, 1), "note": "the headline value"}}
+    Qs = {}
+    nu_o, gain_o, gain_ref = G.DeviceArray((cells, n)), G.DeviceArray((cells, n)), G.DeviceArray((cells, n))
+    for name, fl in forms.items():
+        Q = G.DeviceArray((cells, n))
+        G.check(G.lib().gdt_evaluate(ctx.h, G.METHOD_KOROBOV, f0.ptr, nu_o.ptr,
+                                     (gain_ref if name == "plain_fp64_tiles" else gain_o).ptr, Q.ptr, cells,
+                                     G.LAYOUT_CELL_MAJOR, fl))
+        ctx.sync()
+        Qs[name] = Q.get()
+        fd = G.DeviceArray.from_host(fh)
+        ctx.run_steps(G.METHOD_KOROBOV, fd, cells, DT, 1, fl)  # warm-up (scratch, first launches)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.run_steps(G.METHOD_KOROBOV, fd, cells, DT, steps, fl)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        ctx.kernel_stats_reset()
+        ctx.run_steps(G.METHOD_KOROBOV, fd, cells, DT, 1, fl | G.FLAG_TIME)
+        ctx.sync()
+        ks = {}
+        for k in ("kor_node", "kor_gather", "kor_apply", "ct_pack", "ct_node", "ct_gather", "ct_moments", "ct_solve",
+                  "ct_apply"):
+            ms, cnt = ctx.kernel_stats(k)
+            if cnt:
+                ks[k] = round(1e3 * ms / cells, 4)
+        out[name] = {"evals_per_s": round(cells * steps / el, 1), "kernel_us_per_cell": ks}
+        del fd
+    # per-cell errors: relative to the cell's Q where Q is not negligible
+    # against its loss term (single-Maxwellian cells have Q = 0 to rounding),
+    # and for every cell relative to the loss term's norm |f nu|
+    ref = Qs["plain_fp64_tiles"]
+    lref = (gain_ref.get() - ref)  # gain - Q = f nu (the loss term) of the FP64 tiles
+    refn, ln = np.linalg.norm(ref, axis=1), np.linalg.norm(lref, axis=1)
+    neq = refn > 1e-6 * ln
+    for name, Q in Qs.items():
+        d = np.linalg.norm(Q - ref, axis=1)
+        out[name].update({"rel_l2_Q_vs_fp64": float(np.linalg.norm(Q - ref) / np.linalg.norm(ref)),
+                          "max_cell_rel_l2": float((d[neq] / refn[neq]).max()),
+                          "max_cell_err_vs_loss": float((d / ln).max()),
+                          "conservation_residual": conservation_residual(ctx.xi, Q, ctx.d3v)})
+    out["cells_near_equilibrium"] = int((~neq).sum())
+    for name in ("plain_fp64_grouped", "plain_fp64_tiles", "mixed_fp32_tiles"):
+        e = out[name]["evals_per_s"]
+        out[name]["speedup_vs_plain_fp64_single"] = round(e / single_evals, 2)
+        out[name]["speedup_vs_plain_fp64_grouped"] = round(e / out["plain_fp64_grouped"]["evals_per_s"], 2)
+        out[name]["speedup_vs_plain_fp64_tiles"] = round(e / out["plain_fp64_tiles"]["evals_per_s"], 2)
+    if oracle_cells:
+        import oracle as O
+        g = O.Grid((N_GRID,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+        st = O.korobov_stencil(g, M_NODES, ctx.korobov_info()["a"], S0)
+        lo, q = O.korobov_collide(g, st, fh[:oracle_cells])
+        oq, _, _ = O.correct_loss(g.xi, lo, q, oracle_cells)
+        out["oracle_check"] = {"cells": oracle_cells,
+                               "plain_fp64_tiles_rel_l2": float(np.linalg.norm(ref[:oracle_cells] - oq) / np.linalg.norm(oq)),
+                               "mixed_fp32_tiles_rel_l2": float(np.linalg.norm(Qs["mixed_fp32_tiles"][:oracle_cells] - oq)
+                                                                / np.linalg.norm(oq))}
+    return out
+
+
 def profiled_traffic():
     """HBM bytes per launch per kernel, measured with rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration) and
@@ -365,6 +471,9 @@ def main():
                     help="replay the steps as a hipGraph (GDT_FLAG_GRAPH) instead of launching them on the stream")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the extension legs (uneven batch, matrix cores, surrogate)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the 32^3 many-cell comparison of the plain, tiled and mixed forms")
+    ap.add_argument("--form-cells", type=int, default=2048, help="cells of the 32^3 form comparison")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -466,6 +575,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(ctx.xi, f0, n, ctx.d3v)
 
+    value = args.steps / elapsed
+    forms = None
+    if not args.no_forms:
+        forms = forms_32cube(G, ctx, value, cells=args.form_cells)
+
     ext = {}
     if not args.no_batch:
         ext["uneven_batch"] = uneven_batch(G)
@@ -473,7 +587,6 @@ def main():
         ext["surrogate"] = surrogate_form(G)
         ext["transport"] = transport_step(G)
 
-    value = args.steps / elapsed
     out = {
         "metric": "Q(f,f) evals/s at 32^3 grid: plain FP64 vs mixed/surrogate @ rel-err 1e-4",
         "value": round(value, 3),
@@ -495,6 +608,7 @@ def main():
                    "min_f": float(f_end.min())},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "metric_32cube": forms,
         "extensions": ext,
     }
     print(json.dumps(out), flush=True)

@@ -2,28 +2,33 @@
 //
 // The single-cell path reads every stencil record and entry once per cell and
 // gathers 16-byte node values one request per entry.  Here K cells share each
-// of those reads: a tile holds K cells interleaved [node][K], so one 128-byte
-// row carries a node's value for all K cells (K = 16 in FP64, 32 in FP32), and
-// one stencil record or entry serves K lanes at the same address.
+// of those reads: a tile holds K cells interleaved [node][K], so one row
+// carries a node's value for all K cells, and one stencil record or entry
+// serves the K lanes of a row.  By default K = 64: a wave owns one node (node
+// kernel) or one destination (gather), so records, row offsets and the
+// branch on an entry's role are wave-uniform (scalar loads and scalar
+// address arithmetic); narrow tiles (K = 16 in FP64, 32 in FP32, lane groups
+// of K within a wave) remain as a launch-shape option.
 //
 // Node-term type T: double (the plain method: the same operations as the
 // single-cell node kernel, so the node values are the same bits) or float
-// (the mixed form: f, the node terms D and 2 w P in FP32 with the hardware's
-// log2 / exp2; every sum and the correction in FP64).
+// (the mixed form: f and the node term D in FP32 with the hardware's log2 /
+// exp2; every sum and the correction in FP64).
 //
 // Per evaluation of C cells in T = ceil(C / K) tiles (cells past C are zero):
 //   k_ct_pack     f [cell][node] FP64  ->  ft[tile][node][K] of T
 //   k_ct_node     per chunk of tiles: D of every (node, cell) into
-//                 D[tile][p][K] (p: the sorted node position)
-//   k_ct_gather   per chunk: a group of K lanes per destination walks the
-//                 destination's entries of the global destination-sorted CSR
-//                 in order (one FP64 sum per cell, no partials over node
-//                 groups): Q0[tile][d][K] = sum coef D and the loss
-//                 L[tile][d][K] = f_d sum 2 w f_partner over the entries where
-//                 d is alpha or beta (the loss is linear in f given f_d, so it
-//                 reads the f tile, which stays in L2, instead of a stored
-//                 2 w P); destinations are taken longest first, so the lane
-//                 groups of a wave have similar entry counts
+//                 D[tile][p][K] (p: the sorted node position); r and w come
+//                 precomputed per node
+//   k_ct_gather   per chunk: the K lanes of a destination walk its entries of
+//                 the global destination-sorted CSR in order (one FP64 sum per
+//                 cell, no partials over node groups): Q0[tile][d][K] = sum
+//                 coef D and the loss L[tile][d][K] = f_d sum 2 w f_partner
+//                 over the entries where d is alpha or beta (the loss is
+//                 linear in f given f_d, so it reads the f tile instead of a
+//                 stored 2 w P: half the node kernel's writes and a quarter of
+//                 the gather's reads); destinations are taken longest first,
+//                 so concurrent waves have similar entry counts
 //   k_ct_moments  per (destination range, tile): the 20 moment sums of the
 //                 range, one partial per cell and range
 //   k_ct_solve    per cell: the partials in a fixed order, the 5x5 solve: X
@@ -31,9 +36,15 @@
 //                 corrected nu, f += dt Q, written cell-major through an LDS
 //                 transpose
 // No atomics: every sum has a fixed order, so results are reproducible run to
-// run.  Chunks of tiles bound the node values in flight between the node and
-// gather kernels (ct_chunk_tiles tiles: about 200 MB at 32^3 and 2^21 nodes,
-// so a chunk's node values can stay in the 256 MB Infinity Cache).
+// run and do not depend on the launch shape or on the other cells of a tile.
+// Chunks of tiles bound the node values in flight between the node and
+// gather kernels (4 tiles of 64 cells: 412 MB at 32^3 and 2^21 nodes in FP32).
+//
+// What bounds them (counters at 32^3, 2^21 nodes): both kernels keep the L2
+// channels about 90% busy.  The gather reads each node's D row about 5.7
+// times (once per destination it feeds, scattered over the grid), 80% of
+// those reads miss L2 (about 690 cycles each) and the misses stream 64-byte
+// requests from the memory side at about 3.9 TB/s.
 #include <algorithm>
 
 #include "gdt_internal.h"
@@ -90,20 +101,20 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_pack(int npts, int cells, con
 // node terms
 // ---------------------------------------------------------------------------
 // Packed node record x = (a | b << 15 | D2 >> 10 << 30, l | m << 15,
-// ls | ms << 15, num / 4 | den / 4 << 11 | (D2 & 1023) << 22); fr the tile's
-// f rows, k the lane's cell.
+// ls | ms << 15, num / 4 | den / 4 << 11 | (D2 & 1023) << 22): the six node
+// indices, and num = 0 where r = 0.
 __device__ __forceinline__ int ct_num(uint32_t w) { return (int)(w & 0x7FFu); }
-__device__ __forceinline__ int ct_den(uint32_t w) { return (int)((w >> 11) & 0x7FFu); }
-__device__ __forceinline__ uint32_t ct_d2(const uint4 &x) { return (x.w >> 22) | ((x.x >> 30) << 10); }
 
 // FP64: the operations of the single-cell node kernel (table-driven log and
 // exp from ts, contraction off), so D is the same bits.  ld(i) is the lane's
 // f value of grid node i.
+// r = num / den and w = K sqrt(D2) come precomputed per node (rw: the same
+// correctly rounded values the single-cell kernel forms).
 template <class LD>
-__device__ __forceinline__ double ct_terms(const uint4 x, LD ld, const double *ts, double Kw)
+__device__ __forceinline__ double ct_terms(const uint4 x, const double2 rw, LD ld, const double *ts)
 {
 #pragma clang fp contract(off)
-    const int num = ct_num(x.w), den = ct_den(x.w);
+    const int num = ct_num(x.w);
     const double fa = ld(x.x & 0x7FFFu), fb = ld((x.x >> 15) & 0x7FFFu);
     const double fl = ld(x.y & 0x7FFFu), fm = ld(x.y >> 15);
     const double fls = ld(x.z & 0x7FFFu), fms = ld(x.z >> 15);
@@ -113,15 +124,14 @@ __device__ __forceinline__ double ct_terms(const uint4 x, LD ld, const double *t
     if (num == 0) {
         R = Pl;
     } else {
-        const double r = (double)num / (double)den;
+        const double r = rw.x;
         const double Ps = fls * fms;
         if (Pl > 0.0 && Ps > 0.0)
             R = Pl * kmt_exp(r * kmt_log(Ps / Pl, ts), ts);
         else
             R = (1.0 - r) * Pl + r * Ps;
     }
-    const double w = Kw * sqrt((double)ct_d2(x));
-    return w * (P - R);
+    return rw.y * (P - R);
 }
 
 // FP32: the same formula in single precision with the hardware's
@@ -129,20 +139,20 @@ __device__ __forceinline__ double ct_terms(const uint4 x, LD ld, const double *t
 // quotient lies in (1/2, 2): v_log_f32 never sees a denormal) and an exact
 // integer exponent difference; the quotient is one reciprocal refined by a
 // Newton step (within an ulp of the rounded quotient); r = num / den by the
-// hardware reciprocal (den <= 2047).  R = Pl 2^(r log2(Ps / Pl)); an exponent
-// below -126 gives 0 (a value below 2^-126 Pl, far below the sums' rounding).
-// Products that are not positive take the linear form, as in FP64.
+// r and w come per node, rounded to FP32.  R = Pl 2^(r log2(Ps / Pl)); an
+// exponent below -126 gives 0 (a value below 2^-126 Pl, far below the sums'
+// rounding).  Products that are not positive take the linear form, as in FP64.
 template <class LD>
-__device__ __forceinline__ float ct_terms(const uint4 x, LD ld, const double *ts, float Kw)
+__device__ __forceinline__ float ct_terms(const uint4 x, const float2 rw, LD ld, const double *ts)
 {
     (void)ts;
-    const int num = ct_num(x.w), den = ct_den(x.w);
+    const int num = ct_num(x.w);
     const float fa = ld(x.x & 0x7FFFu), fb = ld((x.x >> 15) & 0x7FFFu);
     const float fl = ld(x.y & 0x7FFFu), fm = ld(x.y >> 15);
     const float fls = ld(x.z & 0x7FFFu), fms = ld(x.z >> 15);
     const float P = fa * fb;
     const float Pl = fl * fm, Ps = fls * fms;
-    const float r = (float)num * __builtin_amdgcn_rcpf((float)den);
+    const float r = rw.x;
     float R = Pl;
     if (num != 0) {
         if (Pl > 0.0f && Ps > 0.0f) {
@@ -157,8 +167,7 @@ __device__ __forceinline__ float ct_terms(const uint4 x, LD ld, const double *ts
             R = (1.0f - r) * Pl + r * Ps;
         }
     }
-    const float w = Kw * __builtin_amdgcn_sqrtf((float)ct_d2(x));
-    return w * (P - R);
+    return rw.y * (P - R);
 }
 
 // Lane group of K lanes: its group index, made uniform (an SGPR) when the
@@ -187,8 +196,20 @@ __device__ __forceinline__ T ct_row(const T *__restrict__ fr, uint32_t i, int k)
 // address per group; the f loads and the value stores are K contiguous
 // elements (a 64- or 128-byte row).
 // Dv holds the chunk's D values [j][p][K].
-template <class T, int K>
-__global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const uint4 *__restrict__ nrec, double Kw,
+template <class T> struct CtRW;
+template <> struct CtRW<double> {
+    typedef double2 type;
+};
+template <> struct CtRW<float> {
+    typedef float2 type;
+};
+
+// Rows hold RS cells; lane groups of K = RS cells.  (Slices of K < RS cells per
+// launch keep a slice of the f tile in L2, but lose the wave-uniform records:
+// 0.90 and 0.98 us per cell for halves and quarters against 0.84 us.)
+template <class T, int K, int RS>
+__global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const uint4 *__restrict__ nrec,
+                                                        const typename CtRW<T>::type *__restrict__ nrw,
                                                         const T *__restrict__ ft, int t0, T *__restrict__ Dv,
                                                         const double *__restrict__ kmt)
 {
@@ -198,26 +219,28 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const 
     __syncthreads();
     constexpr int NPB = CT_THREADS / K;
     const int k = threadIdx.x % K, g = ct_group<K>();
-    const T *fr = ft + (size_t)(t0 + blockIdx.y) * npts * K;
-    T *Dt = Dv + (size_t)blockIdx.y * nn * K;
-    const T kw = (T)Kw;
-    auto ld = [&](uint32_t i) { return ct_row<K>(fr, i, k); };
+    const T *fr = ft + (size_t)(t0 + blockIdx.y) * npts * RS;
+    T *Dt = Dv + (size_t)blockIdx.y * nn * RS;
+    auto ld = [&](uint32_t i) {
+        if (K == WAVE && K == RS) return (fr + (size_t)i * RS)[k];
+        return ld_off(fr, i * RS + k);
+    };
     // two nodes per pass (p and p + stride), evaluated together so that their
     // loads are in flight at once; plain stores: the gather reads the values
     // back soon, and streaming stores would keep them out of the caches on
     // the way to HBM
     auto store = [&](int p, T d) {
-        if (K == WAVE)
-            (Dt + (size_t)p * K)[k] = d;
+        if (K == WAVE && K == RS)
+            (Dt + (size_t)p * RS)[k] = d;
         else
-            *(T *)((char *)Dt + (size_t)(uint32_t)(((uint32_t)p * K + k) * (uint32_t)sizeof(T))) = d;
+            *(T *)((char *)Dt + (size_t)(uint32_t)(((uint32_t)p * RS + k) * (uint32_t)sizeof(T))) = d;
     };
     const int stride = gridDim.x * NPB;
     for (int p = blockIdx.x * NPB + g; p < nn; p += 2 * stride) {
         const int p2 = p + stride < nn ? p + stride : p;  // clamped: in bounds
         const uint4 x = nrec[p], x2 = nrec[p2];
-        const T d = ct_terms(x, ld, ts, kw);
-        const T d2 = ct_terms(x2, ld, ts, kw);
+        const T d = ct_terms(x, nrw[p], ld, ts);
+        const T d2 = ct_terms(x2, nrw[p2], ld, ts);
         store(p, d);
         if (p + stride < nn) store(p2, d2);
     }
@@ -448,8 +471,9 @@ static int ct_launch(gdt_context *h, const double *f, int cells, bool correct, d
     const int nblk = std::max(1, h->ct_node_blocks), gblk = std::max(1, h->ct_gather_blocks);
     for (int t0 = 0; t0 < tiles; t0 += chunk) {
         const int nt = std::min(chunk, tiles - t0);
-        GDT_LAUNCH_TIMED(h, "ct_node", (k_ct_node<T, K>), dim3(nblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
-                         h->kn4.as<uint4>(), h->kK, (const T *)ft, t0, Dv, h->kmath.as<double>());
+        const auto *nrw = (const typename CtRW<T>::type *)(sizeof(T) == 8 ? h->krw.p : h->krwf.p);
+        GDT_LAUNCH_TIMED(h, "ct_node", (k_ct_node<T, K, K>), dim3(nblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
+                         h->kn4.as<uint4>(), nrw, (const T *)ft, t0, Dv, h->kmath.as<double>());
         GDT_LAUNCH_TIMED(h, "ct_gather", (k_ct_gather<T, K>), dim3(gblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
                          h->kcptr.as<int32_t>(), h->kcent.as<uint4>(), h->kdperm.as<int32_t>(), (const T *)Dv,
                          (const T *)ft, t0, Q0, L);
@@ -485,7 +509,8 @@ int ct_config(gdt_context *h, int chunk_tiles, int node_blocks, int gather_block
 int ct_run(gdt_context *h, bool mixed, const double *f, int cells, bool correct, double *Qout, double *Lout,
            double *nu_out, double dt, double *f_upd)
 {
-    GDT_CHECK(h->kcent.p && h->kcptr.p && h->kdperm.p, GDT_E_STATE, "cell tiles: stencil not built");
+    GDT_CHECK(h->kcent.p && h->kcptr.p && h->kdperm.p && h->krw.p && h->krwf.p, GDT_E_STATE,
+              "cell tiles: stencil not built");
     if (h->ct_wide)
         return mixed ? ct_launch<float, WAVE>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd)
                      : ct_launch<double, WAVE>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd);

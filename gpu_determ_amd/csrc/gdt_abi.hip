@@ -730,6 +730,24 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                     cent[4 * (size_t)pos + 3] = (uint32_t)(cb >> 32);
                 }
             }
+            // per sorted position: r and w of the node (FP64, and rounded to
+            // FP32 for the mixed node terms) -- r the quotient num / den the
+            // node kernels form, w = K sqrt(D2) as the lattice kernel formed it
+            std::vector<double> rw(2 * (size_t)nn);
+            std::vector<float> rwf(2 * (size_t)nn);
+            for (long p = 0; p < nn; p++) {
+                const long t = order[p];
+                const uint32_t y = n4[4 * (size_t)p + 3];
+                const double num4 = (double)(y & 0x7FFu), den4 = (double)((y >> 11) & 0x7FFu);
+                rw[2 * p] = num4 == 0.0 ? 0.0 : num4 / den4;
+                rw[2 * p + 1] = cw[t];
+                rwf[2 * p] = (float)rw[2 * p];
+                rwf[2 * p + 1] = (float)rw[2 * p + 1];
+            }
+            GDT_CHECK(h->krw.ensure(sizeof(double) * 2 * nn) == 0 && h->krwf.ensure(sizeof(float) * 2 * nn) == 0,
+                      GDT_E_NOMEM, "gdt_korobov_build: node weight allocation failed");
+            GDT_HIP(hipMemcpy(h->krw.p, rw.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->krwf.p, rwf.data(), sizeof(float) * 2 * nn, hipMemcpyHostToDevice));
             std::vector<int32_t> dperm(np);
             for (int v = 0; v < np; v++) dperm[v] = v;
             std::stable_sort(dperm.begin(), dperm.end(),

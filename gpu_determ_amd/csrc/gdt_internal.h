@@ -98,6 +98,7 @@ struct gdt_context {
     // node positions (uint4 entry records: p << 2 | role, a grid node, an FP64
     // value), and the destinations ordered by entry count, longest first
     DevBuf kcptr, kcent, kdperm;
+    DevBuf krw, krwf;             // (r, w) per sorted node position, FP64 and FP32
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
     int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
     int gain_lds_attr = 0;        // dynamic-LDS bytes the matrix gain kernel is enabled for

@@ -19,7 +19,7 @@
 //   r = |Q - Q_nn|_2 / |Q|_2   (how far the network was from conserving),
 // raised to 1 when the entropy production sum Q log f is positive; cells with
 // r above the threshold are recomputed by the plain solver (the grid's
-// Korobov stencil, corrected).
+// Korobov stencil, corrected, in FP64 cell tiles).
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -323,11 +323,11 @@ struct gdt_surrogate {
     DevBuf X, Xt, T, P1, P2, Hh, Hth, Y, dYh, dYth, dH, dUh, dUf, dUth, dWh, dWf, dWth, gW1, gW2, gW3, gb1, gb2, gb3,
         lossp, idx;
     // inference scratch
-    DevBuf dens, Qnn, res, flg, sub_f, sub_Q, sub_nu, sub_g;
+    DevBuf dens, Qnn, res, flg, sub_f, sub_Q;
     DevBuf gws;  // split-K partials of the skinny GEMMs
 };
 
-#define SG_FALLBACK_CHUNK 2048  // cells per plain-solver call of the fallback
+#define SG_FALLBACK_CHUNK 4096  // cells per plain-solver call of the fallback
 
 static int sg_blocks(long n) { return (int)std::max(1L, (n + 255) / 256); }
 
@@ -474,7 +474,7 @@ int gdt_surrogate_destroy(gdt_surrogate_handle s)
                      &s->W3th, &s->mux, &s->sdx, &s->muy, &s->sdy, &s->X, &s->Xt, &s->T, &s->P1, &s->P2, &s->Hh,
                      &s->Hth, &s->Y, &s->dYh, &s->dYth, &s->dH, &s->dUh, &s->dUf, &s->dUth, &s->dWh, &s->dWf,
                      &s->dWth, &s->gW1, &s->gW2, &s->gW3, &s->gb1, &s->gb2, &s->gb3, &s->lossp, &s->idx, &s->dens,
-                     &s->Qnn, &s->res, &s->flg, &s->sub_f, &s->sub_Q, &s->sub_nu, &s->sub_g, &s->gws};
+                     &s->Qnn, &s->res, &s->flg, &s->sub_f, &s->sub_Q, &s->gws};
     for (DevBuf *d : all) d->release();
     delete s;
     return GDT_OK;
@@ -665,24 +665,25 @@ int gdt_surrogate_infer(gdt_surrogate_handle s, const double *f, double *Q, int 
     if (fallback) GDT_HIP(hipMemcpy(fallback, s->flg.p, 4 * (size_t)cells, hipMemcpyDeviceToDevice));
     if (n_fallback) *n_fallback = k;
     if (k == 0) return GDT_OK;
-    // plain solver on the flagged cells (corrected Korobov evaluation), in
+    // plain solver on the flagged cells: the corrected Korobov operator in
+    // FP64 cell tiles (the many-cell form of the plain method, <= 1e-12 of the
+    // FP64 oracle; a cell's result does not depend on the cells beside it), in
     // chunks of SG_FALLBACK_CHUNK cells so its scratch stays bounded
     GDT_CHECK(h->have_kor, GDT_E_STATE, "gdt_surrogate_infer: %d cells need the plain solver, no stencil built", k);
     const int ch = std::min(k, SG_FALLBACK_CHUNK);
     DevBuf didx;
     GDT_CHECK(didx.ensure(4 * (size_t)k) == 0 && s->sub_f.ensure(8 * (size_t)k * n) == 0 &&
-                  s->sub_Q.ensure(8 * (size_t)k * n) == 0 && s->sub_nu.ensure(8 * (size_t)ch * n) == 0 &&
-                  s->sub_g.ensure(8 * (size_t)ch * n) == 0,
+                  s->sub_Q.ensure(8 * (size_t)k * n) == 0,
               GDT_E_NOMEM, "gdt_surrogate_infer: fallback allocation failed");
+    if ((rc = ct_prepare(h, ch, false))) return rc;
     GDT_HIP(hipMemcpyAsync(didx.p, idx.data(), 4 * (size_t)k, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_sg_rows, dim3(sg_blocks((long)k * n)), dim3(256), 0, st, k, n, (const int *)didx.p, f,
                        s->sub_f.as<double>(), 0);
     GDT_HIP(hipGetLastError());
     for (int c0 = 0; c0 < k; c0 += ch) {
         const int kc = std::min(ch, k - c0);
-        if ((rc = gdt_evaluate(h, GDT_METHOD_KOROBOV, s->sub_f.as<double>() + (size_t)c0 * n, s->sub_nu.as<double>(),
-                               s->sub_g.as<double>(), s->sub_Q.as<double>() + (size_t)c0 * n, kc,
-                               GDT_LAYOUT_CELL_MAJOR, GDT_FLAG_CORRECT)))
+        if ((rc = ct_run(h, false, s->sub_f.as<double>() + (size_t)c0 * n, kc, true,
+                         s->sub_Q.as<double>() + (size_t)c0 * n, nullptr, nullptr, 0.0, nullptr)))
             return rc;
     }
     hipLaunchKernelGGL(k_sg_rows, dim3(sg_blocks((long)k * n)), dim3(256), 0, st, k, n, (const int *)didx.p,

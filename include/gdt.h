@@ -252,7 +252,7 @@ int gdt_surrogate_predict(gdt_surrogate_handle s, const double *f, double *Q, in
 /* Prediction, exact conservation projection (FP64 5x5 per cell, weights f),
  * residual r = |Q - Q_nn| / |Q| (1 if sum Q log f > 0) and, for r >
  * threshold, the plain solver (corrected Korobov evaluation on the stencil of
- * h) in place of the network.  fallback (device int [cells], may be NULL)
+ * h, in FP64 cell tiles) in place of the network.  fallback (device int [cells], may be NULL)
  * marks those cells; *n_fallback counts them.  Synchronous. */
 int gdt_surrogate_infer(gdt_surrogate_handle s, const double *f, double *Q, int cells, double threshold,
                         int *fallback, int *n_fallback);

@@ -162,7 +162,7 @@ def test_surrogate_nonfinite_cells_fall_back():
     assert fb[2] and fb[3]
     assert not fb[0] and not fb[1] and not fb[4]
     assert np.all(np.isfinite(Qh))
-    _, _, Qp = c.evaluate(G.METHOD_KOROBOV, cd, 5, G.FLAG_CORRECT)
+    _, _, Qp = c.evaluate(G.METHOD_KOROBOV, cd, 5, G.FLAG_CORRECT | G.FLAG_CELL_TILES)  # the fallback's solver
     Qp = Qp.get()
     assert np.array_equal(Qh[2], Qp[2]) and np.array_equal(Qh[3], Qp[3])
     assert np.all(Qh[2] == 0.0)

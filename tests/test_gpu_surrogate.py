@@ -81,7 +81,7 @@ def test_infer_projection_and_fallback(setup):
     cells = 96
     x = np.stack([mixture(ctx.xi, rng, comps=1 + (i % 3), Trange=(0.3, 1.5), umax=1.0) for i in range(cells)])
     xd = G.DeviceArray.from_host(x)
-    _, _, Qp = ctx.evaluate(G.METHOD_KOROBOV, xd, cells, G.FLAG_CORRECT)
+    _, _, Qp = ctx.evaluate(G.METHOD_KOROBOV, xd, cells, G.FLAG_CORRECT | G.FLAG_CELL_TILES)  # the fallback's solver
     Qp = Qp.get()
     # generous threshold: the projected network output is kept for most cells
     Q, fb, res = s.infer(xd, threshold=0.5)
