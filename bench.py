@@ -210,96 +210,115 @@ def transport_step(G, shape=(16, 16, 16), steps=20, seed=SEED):
             "alg_bytes": int(alg), "gbs": round(alg / el / 1e9, 1), "frac_of_hbm": round(alg / el / 1e9 / HBM_PEAK_GBS, 3)}
 
 
-def surrogate_form(G, train_cells=200000, infer_cells=16384, epochs=10, threshold=1e-4, seed=SEED):
-    """Learned surrogate on 24^3 (vmax 4.8, sphere cut, 7,208 nodes): trained
-    on `train_cells` (f, Q) pairs from the plain solver (Korobov 2^20,
-    corrected) for seeded mixtures of 1-3 Maxwellians (T in [0.3, 1.5],
-    |u| <= 1); inference on `infer_cells` new cells with the exact projection
-    and plain-solver fallback above `threshold`.  The plain solver judges the
-    network's error on every inference cell."""
-    rng = np.random.default_rng(seed + 5)
-    ctx = G.Context((24,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
-    ctx.korobov_build(1 << 20, seed=seed, s0=S0)
+def plain_tiles(G, ctx, fd, cells, chunk=16384):
+    """Corrected Q of every cell by the plain solver in FP64 cell tiles (the
+    many-cell plain method, within 1e-12 of the oracle), chunk by chunk."""
     n = ctx.n_pnt
+    Q = G.DeviceArray((cells, n))
+    ch = min(chunk, cells)
+    nu, g = G.DeviceArray((ch, n)), G.DeviceArray((ch, n))
+    for c0 in range(0, cells, ch):
+        k = min(ch, cells - c0)
+        G.check(G.lib().gdt_evaluate(ctx.h, G.METHOD_KOROBOV, fd.ptr + 8 * c0 * n, nu.ptr, g.ptr, Q.ptr + 8 * c0 * n,
+                                     k, G.LAYOUT_CELL_MAJOR, G.FLAG_CORRECT | G.FLAG_CELL_TILES))
+    ctx.sync()
+    return Q
 
-    def cells_f(k):
-        """Seeded mixtures as device arrays, built on the GPU from (n, T, u) per component."""
-        comps = rng.integers(1, 4, k)
-        par = np.zeros((k, 3, 5))
-        for j in range(3):
-            u = rng.uniform(-1.0, 1.0, (k, 3))
-            u *= (rng.uniform(0, 1, k) ** (1 / 3) / np.maximum(np.linalg.norm(u, axis=1), 1e-12))[:, None]
-            par[:, j, 0] = rng.uniform(0.2, 1.0, k) * (comps > j)
-            par[:, j, 1] = rng.uniform(0.3, 1.5, k)
-            par[:, j, 2:] = u
-        return ctx.init_mixture(par)
 
-    def plain(fd, k):
-        """Corrected Q of every cell by the plain solver (a batch over the
-        context, segment by segment)."""
-        Q = G.DeviceArray((k, n))
-        b = G.Batch([ctx], [fd], [Q])
-        b.config(1024, 32)
-        b.run(0.0, 1, G.FLAG_CORRECT)
-        b.sync()
-        b.close()
-        return Q, None
-
+def train_surrogates(G, ctx, rng, train_cells, epochs, hidden, lr, seed=SEED):
+    """Two networks (different seeds) trained on `train_cells` (f, Q) pairs
+    of seeded 1-3 Maxwellian mixtures (T in [0.3, 1.5], |u| <= 1) labelled by
+    the plain solver; the second is the first's partner in the residual
+    check."""
     t0 = time.perf_counter()
-    ftr = cells_f(train_cells)
-    Qtr, _ = plain(ftr, train_cells)
+    ftr = ctx.init_mixture(mixture_params(rng, train_cells, umax=1.0))
+    Qtr = plain_tiles(G, ctx, ftr, train_cells)
     gen_s = time.perf_counter() - t0
-    s = G.Surrogate(ctx, hidden=128, seed=seed)
+    nets, losses = [], []
     t0 = time.perf_counter()
-    loss = s.train(ftr, Qtr, epochs=epochs, batch=256, lr=1e-3)
+    for k in range(2):
+        s = G.Surrogate(ctx, hidden=hidden, seed=seed + 101 * k)
+        losses.append(s.train(ftr, Qtr, epochs=epochs, batch=256, lr=lr))
+        nets.append(s)
     train_s = time.perf_counter() - t0
-    del ftr, Qtr
-    fin = cells_f(infer_cells)
-    Qplain, _ = plain(fin, infer_cells)
-    # the plain solver as the fallback runs it (corrected evaluations of 2,048
-    # cells at a time, outputs preallocated), warm, for the throughput
-    # comparison
-    ch = 2048
-    outs = [G.DeviceArray((ch, n)) for _ in range(3)]
+    nets[0].set_partner(nets[1])
+    info = {"train_pairs": train_cells, "epochs": epochs, "hidden": hidden, "lr": lr, "params_per_network": nets[0].params,
+            "networks": 2, "final_loss": losses, "gen_s": round(gen_s, 2), "train_s": round(train_s, 2)}
+    return nets, info
 
-    def plain_chunks():
-        for c0 in range(0, infer_cells, ch):
-            k = min(ch, infer_cells - c0)
-            G.check(G.lib().gdt_evaluate(ctx.h, G.METHOD_KOROBOV, fin.ptr + 8 * c0 * n, outs[0].ptr, outs[1].ptr,
-                                         outs[2].ptr, k, G.LAYOUT_CELL_MAJOR, G.FLAG_CORRECT))
-        ctx.sync()
 
-    plain_chunks()
-    t0 = time.perf_counter()
-    plain_chunks()
-    plain_s = time.perf_counter() - t0
+def surrogate_eval(G, ctx, nets, fin, cells, Qplain, plain_evals, threshold=1e-4):
+    """Surrogate with the plain-solver fallback at `threshold` on `cells`
+    cells: throughput (network pair + projection + fallback), the network's
+    error against the plain solver (which judges every cell), and per
+    threshold the fraction of cells the gate keeps and the true error of the
+    kept cells -- for the pair's disagreement gate and for the projection
+    residual of one network alone."""
+    s = nets[0]
     s.infer(fin, threshold)  # warm-up
     t0 = time.perf_counter()
     Q, fb, res = s.infer(fin, threshold)
     infer_s = time.perf_counter() - t0
-    Qn, _, _ = s.infer(fin, 1e30)  # network + projection only, no fallback
-    qp, qn = Qplain.get(), Qn.get()
-    # per-cell errors relative to the cell's loss-term scale |f nu| would need
-    # nu; cells of one Maxwellian have Q ~ 0, so the error is aggregated over
-    # all cells (sum of squares), and per cell only over non-equilibrium ones
-    agg = float(np.linalg.norm(qn - qp) / np.linalg.norm(qp))
-    qn_norm = np.linalg.norm(qp, axis=1)
-    big = qn_norm > 1e-3 * np.median(qn_norm)
-    errs = np.linalg.norm(qn - qp, axis=1)[big] / qn_norm[big]
     t0 = time.perf_counter()
-    s.infer(fin, 1e30)
+    Qn, _, res_pair = s.infer(fin, 1e30)  # network pair + projection, no fallback
     nn_s = time.perf_counter() - t0
-    out = {"grid": "24^3", "nodes": n, "params": s.params, "train_pairs": train_cells, "epochs": epochs,
-           "final_loss": loss, "gen_s": round(gen_s, 2), "train_s": round(train_s, 2), "infer_cells": infer_cells,
-           "threshold": threshold, "fallback_cells": int(fb.sum()),
-           "evals_per_s": round(infer_cells / infer_s, 1),
-           "network_only_evals_per_s": round(infer_cells / nn_s, 1),
-           "plain_evals_per_s": round(infer_cells / plain_s, 1),
-           "network_rel_l2_vs_plain": agg,
-           "network_cell_rel_l2": {"median": float(np.median(errs)), "p90": float(np.quantile(errs, 0.9)),
-                                   "min": float(errs.min()), "cells": int(big.sum())},
-           "result_rel_l2_vs_plain": float(np.linalg.norm(Q.get() - qp) / np.linalg.norm(qp))}
-    s.close()
+    s.set_partner(None)
+    Q1, _, res_one = s.infer(fin, 1e30)
+    s.set_partner(nets[1])
+    qp, qn, q1 = Qplain.get(), Qn.get(), Q1.get()
+    pn = np.linalg.norm(qp, axis=1)
+    big = pn > 1e-3 * np.median(pn)  # cells away from equilibrium (Q not ~0)
+    err = np.linalg.norm(qn - qp, axis=1)
+    rel = err[big] / pn[big]
+
+    def curve(r, qq):
+        out = []
+        for t in (0.3, 0.1, 0.03, 0.01, 3e-3, 1e-3, 1e-4):
+            keep = r <= t
+            kb = keep[big]
+            e = np.linalg.norm(qq - qp, axis=1)[big] / pn[big]
+            out.append({"threshold": t, "kept_frac": round(float(keep.mean()), 5),
+                        "kept_max_true_rel_l2": float(e[kb].max()) if kb.any() else None,
+                        "kept_median_true_rel_l2": float(np.median(e[kb])) if kb.any() else None})
+        return out
+
+    lr = np.log(np.maximum(res_pair[big], 1e-30))
+    le = np.log(np.maximum(rel, 1e-30))
+    return {"cells": cells, "threshold": threshold, "fallback_cells": int(fb.sum()),
+            "evals_per_s": round(cells / infer_s, 1), "network_only_evals_per_s": round(cells / nn_s, 1),
+            "plain_evals_per_s": round(plain_evals, 1),
+            "speedup_vs_plain": round(cells / infer_s / plain_evals, 3),
+            "result_rel_l2_vs_plain": float(np.linalg.norm(Q.get() - qp) / np.linalg.norm(qp)),
+            "network_rel_l2_vs_plain": float(np.linalg.norm(qn - qp) / np.linalg.norm(qp)),
+            "network_cell_rel_l2": {"median": float(np.median(rel)), "p10": float(np.quantile(rel, 0.1)),
+                                    "min": float(rel.min()), "cells": int(big.sum())},
+            "gate_log_corr_with_true_error": {"pair": float(np.corrcoef(lr, le)[0, 1]),
+                                              "one_network": float(np.corrcoef(
+                                                  np.log(np.maximum(res_one[big], 1e-30)), le)[0, 1])},
+            "curve_pair_gate": curve(res_pair, qn), "curve_one_network_gate": curve(res_one, q1),
+            "conservation_residual": conservation_residual(ctx.xi, Q.get(), ctx.d3v)}
+
+
+def surrogate_form(G, train_cells=200000, infer_cells=16384, epochs=40, hidden=512, lr=5e-4, threshold=1e-4,
+                   seed=SEED):
+    """Learned surrogate on 24^3 (vmax 4.8, sphere cut, 7,208 nodes, Korobov
+    2^20): two networks trained on `train_cells` plain-solver pairs, inference
+    on `infer_cells` new cells with the exact projection and the plain-solver
+    fallback above `threshold`."""
+    rng = np.random.default_rng(seed + 5)
+    ctx = G.Context((24,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+    ctx.korobov_build(1 << 20, seed=seed, s0=S0)
+    nets, info = train_surrogates(G, ctx, rng, train_cells, epochs, hidden, lr, seed)
+    fin = ctx.init_mixture(mixture_params(rng, infer_cells, umax=1.0))
+    Qplain = plain_tiles(G, ctx, fin, infer_cells)
+    t0 = time.perf_counter()
+    plain_tiles(G, ctx, fin, infer_cells)
+    plain_evals = infer_cells / (time.perf_counter() - t0)
+    out = {"grid": "24^3", "nodes": ctx.n_pnt}
+    out.update(info)
+    out.update(surrogate_eval(G, ctx, nets, fin, infer_cells, Qplain, plain_evals, threshold))
+    for s_ in nets:
+        s_.close()
     return out
 
 
@@ -326,7 +345,8 @@ def conservation_residual(xi, Q, d3v):
     return float(np.max(m / np.maximum(sc, 1e-300)))
 
 
-def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed=SEED):
+def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed=SEED, surrogate=True,
+                 surrogate_train=100000, surrogate_epochs=20):
     """The metric's comparison on one set of cells at 32^3 (vmax 4.8, 17,256
     nodes, 2^21 lattice nodes, s0 = 0.25): `cells` seeded mixtures of 1-3
     Maxwellians.  Throughput unit: one corrected Q(f,f) evaluation with the
@@ -336,6 +356,10 @@ def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed
       plain_fp64_tiles    FP64 cell tiles (64 cells per row, same node terms)
       mixed_fp32_tiles    cell tiles with f and the node terms in FP32, sums
                           and the 5x5 correction in FP64
+      surrogate_with_fallback  two networks trained on `surrogate_train`
+                          plain-solver pairs of 32^3 cells, the exact
+                          projection, and the plain solver (FP64 tiles) for
+                          every cell whose residual exceeds 1e-4
     Accuracy on the unstepped cells: corrected Q against the FP64 tiles over
     every cell (the FP64 tiles are within 1e-12 of the oracle; checked here on
     `oracle_cells` cells too), the worst cell, and the conservation residual
@@ -396,6 +420,20 @@ def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed
         out[name]["speedup_vs_plain_fp64_single"] = round(e / single_evals, 2)
         out[name]["speedup_vs_plain_fp64_grouped"] = round(e / out["plain_fp64_grouped"]["evals_per_s"], 2)
         out[name]["speedup_vs_plain_fp64_tiles"] = round(e / out["plain_fp64_tiles"]["evals_per_s"], 2)
+    if surrogate:
+        # surrogate with the plain-solver fallback on the same cells (the
+        # plain solver of the fallback: the FP64 tiles)
+        srng = np.random.default_rng(seed + 8)
+        # at 32^3 a rate of 1e-3 diverges; 3e-4 over 100k pairs reaches ~0.26
+        # relative error, 256 hidden units as well as 512
+        nets, info = train_surrogates(G, ctx, srng, surrogate_train, surrogate_epochs, 256, 3e-4, seed)
+        sg = {"train": info}
+        sg.update(surrogate_eval(G, ctx, nets, f0, cells, G.DeviceArray.from_host(ref),
+                                 out["plain_fp64_tiles"]["evals_per_s"], 1e-4))
+        sg["speedup_vs_plain_fp64_single"] = round(sg["evals_per_s"] / single_evals, 2)
+        out["surrogate_with_fallback"] = sg
+        for s_ in nets:
+            s_.close()
     if oracle_cells:
         import oracle as O
         g = O.Grid((N_GRID,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)

@@ -49,7 +49,7 @@ ABI_SYMBOLS = (
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
     "gdt_init_mixture", "gdt_korobov_tiles", "gdt_korobov_set_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
-    "gdt_transport", "gdt_korobov_cell_tiles", "gdt_evaluate_noalloc",
+    "gdt_transport", "gdt_korobov_cell_tiles", "gdt_evaluate_noalloc", "gdt_surrogate_set_partner",
 )
 
 _lib = None
@@ -128,6 +128,7 @@ def lib():
         "gdt_surrogate_predict": [vp, vp, vp, C.c_int],
         "gdt_surrogate_infer": [vp, vp, vp, C.c_int, C.c_double, vp, ip],
         "gdt_surrogate_residuals": [vp, vp, C.c_int],
+        "gdt_surrogate_set_partner": [vp, vp],
         "gdt_surrogate_export": [vp, vp, vp, vp, vp, vp, vp, vp],
         "gdt_surrogate_info": [vp, ip, ip, ip, C.POINTER(C.c_long)],
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
@@ -473,6 +474,11 @@ class Surrogate:
         check(lib().gdt_surrogate_train(self.h, f.ptr, Q.ptr, int(f.shape[0]), int(epochs), int(batch), float(lr),
                                         C.byref(loss)))
         return loss.value
+
+    def set_partner(self, other):
+        """Second network of the residual check (None clears)."""
+        check(lib().gdt_surrogate_set_partner(self.h, other.h if other is not None else None))
+        self.partner = other
 
     def predict(self, f):
         Q = DeviceArray(f.shape)

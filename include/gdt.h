@@ -250,12 +250,18 @@ int gdt_surrogate_train(gdt_surrogate_handle s, const double *f, const double *Q
 /* Network prediction only (no projection): Q [cells][n_pnt] device FP64. */
 int gdt_surrogate_predict(gdt_surrogate_handle s, const double *f, double *Q, int cells);
 /* Prediction, exact conservation projection (FP64 5x5 per cell, weights f),
- * residual r = |Q - Q_nn| / |Q| (1 if sum Q log f > 0) and, for r >
+ * residual r = |Q - Q_nn| / |Q| (at least 1 if sum Q log f > 0) and, for r >
  * threshold, the plain solver (corrected Korobov evaluation on the stencil of
  * h, in FP64 cell tiles) in place of the network.  fallback (device int [cells], may be NULL)
  * marks those cells; *n_fallback counts them.  Synchronous. */
 int gdt_surrogate_infer(gdt_surrogate_handle s, const double *f, double *Q, int cells, double threshold,
                         int *fallback, int *n_fallback);
+/* Second network for the residual check of s (same context; NULL clears):
+ * gdt_surrogate_infer then predicts with both, keeps their mean, and raises
+ * each cell's residual to |Q_a - Q_b| / |Q| (their disagreement, an estimate
+ * of the error against the plain solver).  The partner must outlive s's use
+ * of it. */
+int gdt_surrogate_set_partner(gdt_surrogate_handle s, gdt_surrogate_handle partner);
 /* Host copy of the residuals of the last gdt_surrogate_infer call. */
 int gdt_surrogate_residuals(gdt_surrogate_handle s, double *res, int cells);
 /* Host copies of the FP32 master weights ([hidden][np], [hidden],
