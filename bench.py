@@ -113,8 +113,40 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
     b.run(DT, steps, G.FLAG_CORRECT)
     b.sync()
     el_eager = time.perf_counter() - t0
+    # the same steps with every class's cells in cell tiles (FP64: the same
+    # plain method; mixed: FP32 node terms), graph replay
+    tiles = {}
+    for name, fl in (("fp64_tiles", G.FLAG_CELL_TILES), ("mixed_fp32_tiles", G.FLAG_MIXED)):
+        fl = G.FLAG_CORRECT | G.FLAG_GRAPH | fl
+        b.run(DT, 1, fl)  # scratch, capture, warm-up
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(DT, steps, fl)
+        b.sync()
+        e = time.perf_counter() - t0
+        tiles[name] = {"evals_per_s": round(cells * steps / e, 1), "ms_per_step": round(1e3 * e / steps, 3)}
+    # roofline per the algorithmic bytes of one evaluation: f in + Q out (16 N)
+    # plus the class stencil over the cells sharing one pass of it -- one
+    # cell for the work lists (16-byte node records, 8-byte entries), 64
+    # cells for the tiles (node records and FP64 (r, w) 32 B per node, 16-byte
+    # entries)
+    alg_wl = alg_t = 0.0
+    for i, c in enumerate(ctxs):
+        k = int(np.sum(pick == i))
+        inf = c.korobov_info()
+        alg_wl += k * (16.0 * c.n_pnt + 16.0 * inf["nodes"] + 8.0 * inf["entries"])
+        alg_t += k * (16.0 * c.n_pnt + (32.0 * inf["nodes"] + 16.0 * inf["entries"]) / 64.0)
+
+    def roof(alg, ms):
+        gbs = alg / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "alg_bytes_per_step": int(alg), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    tiles["fp64_tiles"]["roofline"] = roof(alg_t, tiles["fp64_tiles"]["ms_per_step"])
+    tiles["mixed_fp32_tiles"]["roofline"] = roof(alg_t, tiles["mixed_fp32_tiles"]["ms_per_step"])
     out = {"cells": cells, "grids": {"%d^3" % n: int(np.sum(pick == i)) for i, n in enumerate(sizes)},
            "korobov_M": 1 << 20, "steps": steps, "evals_per_s": round(cells * steps / el, 1),
+           "roofline": roof(alg_wl, 1e3 * el / steps), "cell_tiles": tiles,
            "ms_per_step": round(1e3 * el / steps, 3), "eager_evals_per_s": round(cells * steps / el_eager, 1),
            "tuned": {"seg_cells": seg, "nodes_per_thread": q, "best_eval_ms": round(tune_ms, 3),
                      "gather_tiles_per_group": {"%d^3" % n: c.korobov_tiles()["tiles"] for n, c in zip(sizes, ctxs)}},

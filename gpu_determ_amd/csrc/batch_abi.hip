@@ -1,7 +1,8 @@
 // Uneven batches of the Korobov method (gdt_batch_* in include/gdt.h): many
 // cells on several velocity grids, one corrected evaluation (or explicit
 // step) of every cell per run step, as three work-list kernels per segment of
-// cells, the whole step captured as one hipGraph.
+// cells, or (GDT_FLAG_CELL_TILES / GDT_FLAG_MIXED) each class's cells in cell
+// tiles; the whole step captured as one hipGraph.
 //
 // A segment holds up to seg_cells cells (of any grids, in class order); its
 // scratch (node values, group partials, moment sums) is reused by the next
@@ -41,6 +42,7 @@ struct gdt_batch {
     hipGraphExec_t graph = nullptr;
     double g_dt = 0.0;
     unsigned g_flags = 0;
+    unsigned long g_gen = 0;
     bool g_valid = false;
 };
 
@@ -202,10 +204,42 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
     return GDT_OK;
 }
 
-// One run step over every segment on the batch stream.
+// Launches of a class context redirected onto the batch stream for the
+// scope (the cell-tile launchers use the context's stream).
+struct ClassStream {
+    gdt_context *h;
+    hipStream_t old;
+    ClassStream(gdt_context *h_, hipStream_t s) : h(h_), old(h_->stream) { h->stream = s; }
+    ~ClassStream() { h->stream = old; }
+};
+
+static bool batch_tiles(unsigned flags) { return (flags & (GDT_FLAG_CELL_TILES | GDT_FLAG_MIXED)) != 0u; }
+
+// Cell-tile scratch of every class for all its cells (outside any capture).
+static int batch_tiles_prepare(gdt_batch *b, unsigned flags)
+{
+    for (size_t s = 0; s < b->cls.size(); s++)
+        if (b->cells[s] > 0)
+            if (int rc = ct_prepare(b->cls[s], b->cells[s], (flags & GDT_FLAG_MIXED) != 0u)) return rc;
+    return GDT_OK;
+}
+
+// One run step over every segment on the batch stream: the work lists, or,
+// with GDT_FLAG_CELL_TILES / GDT_FLAG_MIXED, every class's cells in cell
+// tiles (each class's stencil read once per tile of 64 cells).
 static int batch_step(gdt_batch *b, double dt, unsigned flags)
 {
     const bool corr = (flags & GDT_FLAG_CORRECT) != 0;
+    if (batch_tiles(flags)) {
+        for (size_t s = 0; s < b->cls.size(); s++) {
+            if (b->cells[s] == 0) continue;
+            ClassStream cs(b->cls[s], b->stream);
+            int rc = ct_run(b->cls[s], (flags & GDT_FLAG_MIXED) != 0u, b->f[s], b->cells[s], corr,
+                            b->Q.empty() ? nullptr : b->Q[s], nullptr, nullptr, dt, dt != 0.0 ? b->f[s] : nullptr);
+            if (rc) return rc;
+        }
+        return GDT_OK;
+    }
     for (auto &sg : b->segs) {
         const KorClassDev *d = sg.desc.as<KorClassDev>();
         int rc;
@@ -372,11 +406,15 @@ int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags)
 {
     B_CHECK(b);
     GDT_CHECK(steps >= 0, GDT_E_ARG, "gdt_batch_run: steps %d", steps);
-    if (!b->seg_cells) {
-        int rc = batch_configure(b, 256, 2);
-        if (rc) return rc;
+    if (batch_tiles(flags)) {
+        if (int rc = batch_tiles_prepare(b, flags)) return rc;
+    } else {
+        if (!b->seg_cells) {
+            int rc = batch_configure(b, 256, 2);
+            if (rc) return rc;
+        }
+        if (int rc = batch_check_classes(b)) return rc;
     }
-    if (int rc = batch_check_classes(b)) return rc;
     if (!(flags & GDT_FLAG_GRAPH)) {
         for (int s = 0; s < steps; s++) {
             int rc = batch_step(b, dt, flags);
@@ -384,7 +422,10 @@ int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags)
         }
         return GDT_OK;
     }
-    if (!b->g_valid || b->g_dt != dt || b->g_flags != flags) {
+    // the captured launches hold the scratch pointers of the work lists or of
+    // the classes' cell tiles: recapture after any buffer was freed
+    const unsigned long gen = devbuf_generation();
+    if (!b->g_valid || b->g_dt != dt || b->g_flags != flags || b->g_gen != gen) {
         drop_graph(b);
         hipGraph_t g = nullptr;
         GDT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -400,6 +441,7 @@ int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags)
         GDT_HIP(e);
         b->g_dt = dt;
         b->g_flags = flags;
+        b->g_gen = devbuf_generation();
         b->g_valid = true;
     }
     for (int s = 0; s < steps; s++) GDT_HIP(hipGraphLaunch(b->graph, b->stream));

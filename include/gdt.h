@@ -277,8 +277,10 @@ int gdt_surrogate_info(gdt_surrogate_handle s, int *n, int *np, int *hidden, lon
  * [cells[s]][n_pnt of class s] (cell-major), updated in place by steps.
  * Each run step evaluates every cell (corrected if GDT_FLAG_CORRECT) and,
  * for dt != 0, applies f += dt Q: per segment of cells, three work-list
- * kernels; with GDT_FLAG_GRAPH the whole step is one hipGraph.
- * Every cell's result is bitwise that of gdt_run_steps on the cell alone.
+ * kernels, or with GDT_FLAG_CELL_TILES / GDT_FLAG_MIXED every class's cells
+ * in cell tiles (each class's stencil read once per tile of 64 cells); with
+ * GDT_FLAG_GRAPH the whole step is one hipGraph.  Every cell's result is
+ * bitwise that of gdt_run_steps on the cell alone with the same flags.
  * The batch keeps the f / Q pointers and the class handles: they must
  * outlive it. */
 typedef struct gdt_batch *gdt_batch_handle;

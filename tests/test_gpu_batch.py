@@ -175,3 +175,50 @@ def test_batch_uneven_grids_vs_oracle():
     for ctx, x0, d in zip(classes, f0, fd):
         assert np.array_equal(d.get(), single_runs(ctx, x0, 1, G.FLAG_CORRECT))
     b.close()
+
+
+def test_batch_cell_tiles_mode():
+    """Batches in cell tiles (GDT_FLAG_CELL_TILES, and the mixed form): the
+    uneven-batch grid classes at 2^20 nodes, with a class of 70 cells (two
+    tiles, one ragged).  FP64 tiles match the oracle to 1e-12, the mixed form
+    to 1e-5; an explicit step equals single-cell steps with the same flags bit
+    for bit; graph replay equals eager launches."""
+    import oracle as O
+    from helpers import rel_l2
+    M = 1 << 20
+    sizes = (16, 20, 24, 32)
+    cells = [70, 3, 2, 3]
+    print("batch in cell tiles", flush=True)
+    classes = make_classes(sizes, M=M, seed=7)
+    rng = np.random.default_rng(19)
+    f0 = [cell_fs(c, k, rng) for c, k in zip(classes, cells)]
+    for fl, tol in ((G.FLAG_CELL_TILES, 1e-12), (G.FLAG_MIXED, 1e-5)):
+        fd = [G.DeviceArray.from_host(x) for x in f0]
+        qd = [G.DeviceArray((k, c.n_pnt)) for c, k in zip(classes, cells)]
+        b = G.Batch(classes, fd, qd)
+        b.run(0.0, 1, G.FLAG_CORRECT | fl)
+        b.sync()
+        for n, ctx, x0, q in zip(sizes, classes, f0, qd):
+            g = O.Grid((n,) * 3, (-VMAX,) * 3, (VMAX,) * 3, VMAX)
+            st = O.korobov_stencil(g, M, ctx.korobov_info()["a"], 0.25)
+            k = min(x0.shape[0], 4)
+            lo, q0 = O.korobov_collide(g, st, x0[:k])
+            oq, _, _ = O.correct_loss(g.xi, lo, q0, k)
+            err = rel_l2(q.get()[:k], oq)
+            print("  flags %d %d^3: rel L2 vs oracle %.3g" % (fl, n, err), flush=True)
+            assert err < tol
+        b.run(DT, 1, G.FLAG_CORRECT | fl | G.FLAG_GRAPH)
+        b.sync()
+        for ctx, x0, d in zip(classes, f0, fd):
+            assert np.array_equal(d.get(), single_runs(ctx, x0, 1, G.FLAG_CORRECT | fl))
+        b.run(DT, 1, G.FLAG_CORRECT | fl)  # eager after the graph: the same second step
+        b.sync()
+        e1 = [d.get() for d in fd]
+        fd2 = [G.DeviceArray.from_host(x) for x in f0]
+        b2 = G.Batch(classes, fd2)
+        b2.run(DT, 2, G.FLAG_CORRECT | fl | G.FLAG_GRAPH)
+        b2.sync()
+        for u, v in zip(e1, fd2):
+            assert np.array_equal(u, v.get())
+        b.close()
+        b2.close()
