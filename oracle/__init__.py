@@ -13,6 +13,7 @@ _LIB = None
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
@@ -36,7 +37,24 @@ def lib():
     path = os.path.join(_HERE, "liboracle.so")
     if not os.path.exists(path):
         raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
-    L = C.CDLL(path)
+    _LIB = _bind(C.CDLL(path))
+    return _LIB
+
+
+def use_native_build(outdir):
+    """Compile the oracle for this host's CPU (gcc -march=native, the Makefile's
+    `native` target) into outdir and make it the library every function here
+    calls.  For the CPU baseline, which is timed on the machine that runs it.
+    Returns the library path."""
+    global _LIB
+    import subprocess
+    out = os.path.join(os.path.abspath(outdir), "liboracle_native.so")
+    subprocess.run(["make", "-s", "-C", _HERE, "native", "OUT=" + out], check=True)
+    _LIB = _bind(C.CDLL(out))
+    return out
+
+
+def _bind(L):
     i3 = C.POINTER(C.c_int)
     d3 = C.POINTER(C.c_double)
     L.orc_grid_nodes.argtypes = [i3, d3, d3, C.c_double, _i32p, _ptr_or_null(_i32p)]
@@ -73,7 +91,9 @@ def lib():
                                       _ptr_or_null(_u32p), C.c_long]
     L.orc_korobov_stencil.restype = C.c_long
     L.orc_korobov_collide.argtypes = [C.c_int, C.c_long, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int]
-    _LIB = L
+    L.orc_korobov_dest_lists.argtypes = [C.c_int, C.c_long, _i32p, _f64p, _i64p, _i32p, _f64p, _i64p, _i32p]
+    L.orc_korobov_collide_dest.argtypes = [C.c_int, C.c_long, _i32p, _f64p, _f64p, _i64p, _i32p, _f64p, _i64p,
+                                           _i32p, _f64p, _f64p, _f64p, C.c_int]
     return L
 
 
@@ -244,6 +264,34 @@ def korobov_collide(g, st, f_cmp):
     Q = np.empty(cells * g.n_pnt)
     lib().orc_korobov_collide(g.n_pnt, idx6.shape[0], np.ascontiguousarray(idx6).ravel(), r, w,
                               f_cmp.ravel(), lo, Q, cells)
+    return lo.reshape(cells, g.n_pnt), Q.reshape(cells, g.n_pnt)
+
+
+def korobov_dest_lists(g, st):
+    """Per-destination write lists of a stencil, in the serial scatter's
+    order: (qrow, qnode, qcoef) for Q and (lrow, lnode) for the loss; input
+    of korobov_collide_dest."""
+    idx6 = np.ascontiguousarray(st[0], dtype=np.int32).ravel()
+    r = np.ascontiguousarray(st[1], dtype=np.float64)
+    nn = idx6.size // 6
+    qrow = np.empty(g.n_pnt + 1, np.int64)
+    lrow = np.empty(g.n_pnt + 1, np.int64)
+    qnode = np.empty(max(6 * nn, 1), np.int32)
+    qcoef = np.empty(max(6 * nn, 1), np.float64)
+    lnode = np.empty(max(2 * nn, 1), np.int32)
+    lib().orc_korobov_dest_lists(g.n_pnt, nn, idx6, r, qrow, qnode, qcoef, lrow, lnode)
+    return qrow, qnode, qcoef, lrow, lnode
+
+
+def korobov_collide_dest(g, st, lists, f_cmp):
+    """korobov_collide with OpenMP over destinations (same bits)."""
+    idx6, r, w, _ = st
+    f_cmp = np.ascontiguousarray(f_cmp, dtype=np.float64)
+    cells = f_cmp.size // g.n_pnt
+    lo = np.empty(cells * g.n_pnt)
+    Q = np.empty(cells * g.n_pnt)
+    lib().orc_korobov_collide_dest(g.n_pnt, idx6.shape[0], np.ascontiguousarray(idx6).ravel(), r, w, *lists,
+                                   f_cmp.ravel(), lo, Q, cells)
     return lo.reshape(cells, g.n_pnt), Q.reshape(cells, g.n_pnt)
 
 

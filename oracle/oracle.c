@@ -829,3 +829,88 @@ void orc_korobov_collide(int npts, long nn, const int32_t *idx6,
     }
     free(val);
 }
+
+/* Destination lists of a stencil, for the CPU baseline's parallel form of
+ * orc_korobov_collide.  For every compact velocity v, the writes of the
+ * serial scatter above to q[v] and to loss[v], in the order it makes them
+ * (node t ascending, slot k = 0..5 within a node): qnode[e] = t with
+ * qcoef[e] = -1 (k = 0, 1), 1 - r[t] (k = 2, 3) or r[t] (k = 4, 5), for q;
+ * lnode[e] = t (k = 0, 1) for the loss.  qrow and lrow have npts + 1
+ * entries; qnode/qcoef hold 6 nn entries, lnode 2 nn.  Built once per
+ * stencil. */
+void orc_korobov_dest_lists(int npts, long nn, const int32_t *idx6, const double *r,
+                            long *qrow, int32_t *qnode, double *qcoef, long *lrow, int32_t *lnode)
+{
+    for (int v = 0; v <= npts; v++) qrow[v] = lrow[v] = 0;
+    for (long e = 0; e < 6 * nn; e++) {
+        qrow[idx6[e] + 1]++;
+        if (e % 6 < 2) lrow[idx6[e] + 1]++;
+    }
+    for (int v = 0; v < npts; v++) {
+        qrow[v + 1] += qrow[v];
+        lrow[v + 1] += lrow[v];
+    }
+    long *fq = malloc(sizeof(long) * 2 * (npts > 0 ? npts : 1)), *fl = fq + npts;
+    for (int v = 0; v < npts; v++) {
+        fq[v] = qrow[v];
+        fl[v] = lrow[v];
+    }
+    for (long t = 0; t < nn; t++) {
+        for (int k = 0; k < 6; k++) {
+            const int v = idx6[6 * t + k];
+            const long e = fq[v]++;
+            qnode[e] = (int32_t)t;
+            qcoef[e] = k < 2 ? -1.0 : (k < 4 ? 1.0 - r[t] : r[t]);
+            if (k < 2) lnode[fl[v]++] = (int32_t)t;
+        }
+    }
+    free(fq);
+}
+
+/* orc_korobov_collide with the scatter turned into a gather over
+ * destinations (OpenMP over v) through orc_korobov_dest_lists.  Each
+ * destination adds its writes in the serial scatter's order with the same
+ * operations (q - wd is q + (-1) wd exactly; the (1 - r) and r factors are
+ * the same products), so loss and Q are bitwise those of
+ * orc_korobov_collide. */
+void orc_korobov_collide_dest(int npts, long nn, const int32_t *idx6,
+                              const double *r, const double *w,
+                              const long *qrow, const int32_t *qnode, const double *qcoef,
+                              const long *lrow, const int32_t *lnode,
+                              const double *f_cmp, double *loss, double *Q,
+                              int cells)
+{
+    double *val = malloc(sizeof(double) * 2 * (nn > 0 ? nn : 1));
+    for (int c = 0; c < cells; c++) {
+        const double *f = f_cmp + (long)c * npts;
+        double *L = loss + (long)c * npts, *q = Q + (long)c * npts;
+#pragma omp parallel for schedule(static)
+        for (long t = 0; t < nn; t++) {
+            const int32_t *o = idx6 + 6 * t;
+            const double P = f[o[0]] * f[o[1]];
+            const double Pl = f[o[2]] * f[o[3]];
+            const double rr = r[t];
+            double R;
+            if (rr == 0.0) {
+                R = Pl;
+            } else {
+                const double Ps = f[o[4]] * f[o[5]];
+                if (Pl > 0.0 && Ps > 0.0)
+                    R = Pl * exp(rr * log(Ps / Pl));
+                else
+                    R = (1.0 - rr) * Pl + rr * Ps;
+            }
+            val[2 * t] = w[t] * (P - R);
+            val[2 * t + 1] = 2.0 * (w[t] * P);
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int v = 0; v < npts; v++) {
+            double qa = 0.0, la = 0.0;
+            for (long e = qrow[v]; e < qrow[v + 1]; e++) qa += qcoef[e] * val[2 * (long)qnode[e]];
+            for (long e = lrow[v]; e < lrow[v + 1]; e++) la += val[2 * (long)lnode[e] + 1];
+            q[v] = qa;
+            L[v] = la;
+        }
+    }
+    free(val);
+}

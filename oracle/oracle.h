@@ -70,6 +70,14 @@ void orc_korobov_collide(int npts, long nn, const int32_t *idx6,
                          const double *r, const double *w,
                          const double *f_cmp, double *loss, double *Q,
                          int cells);
+void orc_korobov_dest_lists(int npts, long nn, const int32_t *idx6, const double *r,
+                            long *qrow, int32_t *qnode, double *qcoef, long *lrow, int32_t *lnode);
+void orc_korobov_collide_dest(int npts, long nn, const int32_t *idx6,
+                              const double *r, const double *w,
+                              const long *qrow, const int32_t *qnode, const double *qcoef,
+                              const long *lrow, const int32_t *lnode,
+                              const double *f_cmp, double *loss, double *Q,
+                              int cells);
 
 #ifdef __cplusplus
 }

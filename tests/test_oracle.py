@@ -382,6 +382,36 @@ def test_korobov_gain_approaches_matrix_method():
     assert err < 0.15 and gerr < 0.08
 
 
+def test_korobov_dest_gather_bitwise(kor20):
+    """The CPU baseline's form (OpenMP over destinations through per-
+    destination write lists) gives the serial scatter's bits, on one cell and
+    on three; the lists are in scatter order and cover every write once."""
+    g, M, a, st = kor20
+    idx6, r = st[0], st[1]
+    nn = idx6.shape[0]
+    lists = O.korobov_dest_lists(g, st)
+    qrow, qnode, qcoef, lrow, lnode = lists
+    assert qrow[-1] == 6 * nn and lrow[-1] == 2 * nn
+    dest = np.repeat(np.arange(g.n_pnt), np.diff(qrow))
+    # the lists hold exactly the scatter's (node, destination, factor) writes
+    t = np.repeat(np.arange(nn), 6)
+    k = np.tile(np.arange(6), nn)
+    want = np.stack([t, idx6.ravel(), np.where(k < 2, -1.0, np.where(k < 4, 1.0 - r[t], r[t]))])
+    got = np.stack([qnode, dest, qcoef])
+    assert np.array_equal(want[:, np.lexsort(want[::-1])], got[:, np.lexsort(got[::-1])])
+    # within a destination the nodes never go backwards (scatter order)
+    assert np.all((np.diff(qnode) >= 0) | (np.diff(dest) != 0))
+    ldest = np.repeat(np.arange(g.n_pnt), np.diff(lrow))
+    assert np.array_equal(np.sort(lnode.astype(np.int64) * g.n_pnt + ldest),
+                          np.sort(np.concatenate([np.arange(nn) * g.n_pnt + idx6[:, k] for k in (0, 1)])))
+    rng = np.random.default_rng(12)
+    for cells in (1, 3):
+        f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
+        lo, q = O.korobov_collide(g, st, f)
+        lo2, q2 = O.korobov_collide_dest(g, st, lists, f)
+        assert np.array_equal(lo, lo2) and np.array_equal(q, q2)
+
+
 def test_korobov_correction_conserves(kor20):
     g, M, a, st = kor20
     rng = np.random.default_rng(4)
