@@ -103,3 +103,54 @@ def test_driver_corrected_run_conserves(tmp_path):
     # Q's moments: corrected, it shrinks with them
     assert np.all(np.isfinite(hc)) and np.all(np.isfinite(hu)) and hc[0] == hu[0]
     assert abs(hc[1]) < abs(hu[1])
+
+
+# ------------------------------------------- a reference-shaped caller --
+CALLER = os.path.join(os.path.dirname(__file__), "callers", "reference_shaped_caller.cpp")
+PKG = os.path.join(ROOT, "gpu_determ_amd")
+
+
+def build_caller(outdir):
+    """Compile and link the reference-shaped host program against the drop-in
+    headers and libraries (the link fails on any entry point the drop-in
+    lacks)."""
+    exe = os.path.join(str(outdir), "caller")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(ROOT, "include", "gpu_determ"), CALLER, "-o", exe, "-L" + PKG, "-lgpu_determ",
+           "-lgdt", "-Wl,-rpath," + PKG]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_reference_shaped_caller_builds(tmp_path):
+    exe = build_caller(tmp_path)
+    assert os.access(exe, os.X_OK)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("corr", [0, 1])
+def test_reference_shaped_caller_runs(tmp_path, corr):
+    """The reference-shaped program reads ./gpu.conf and writes ./results.out
+    as determ.cu does; its rows equal the drop-in driver's on the same
+    configuration, and without the correction they reproduce the golden
+    rows of the reference's own run."""
+    exe = build_caller(tmp_path)
+    (tmp_path / "gpu.conf").write_text("nx = 16\nny = 16\nnz = 16\nnpx = 256\ncorrection = %d\n" % corr)
+    r = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0 and "END_PROGRAM" in r.stdout
+    assert "M0=" in r.stdout
+    mine = np.loadtxt(tmp_path / "results.out")
+    r2 = subprocess.run([DETERM, str(tmp_path / "gpu.conf"), str(tmp_path / "driver.out")], capture_output=True,
+                        text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    assert (tmp_path / "results.out").read_text() == (tmp_path / "driver.out").read_text()
+    gold = np.genfromtxt(os.path.join(GOLD, "reference_rows.tsv"), names=True, delimiter="\t")
+    assert mine.shape == (512, 9)
+    assert np.max(np.abs(mine[:, 4] - gold["F"]) / gold["F"]) < 1e-5
+    rel = np.abs(mine[:, 6] - gold["IC"]) / gold["IC"]
+    assert np.max(rel[gold["race"] == 0]) < 1e-4
+    if corr == 0:
+        keep = gold["cell0"] == 0
+        assert np.max(np.abs(mine[keep, 5] - gold["DC"][keep]) / gold["DC"][keep]) < 1e-5
