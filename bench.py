@@ -448,6 +448,18 @@ def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed
                           "max_cell_err_vs_loss": float((d / ln).max()),
                           "conservation_residual": conservation_residual(ctx.xi, Q, ctx.d3v)})
     out["cells_near_equilibrium"] = int((~neq).sum())
+    # algorithmic bytes of one evaluation: f in and f out (16 N) plus the
+    # stencil over the cells that share one pass of it -- one cell for the
+    # grouped kernels (16-byte node records, 8-byte entries), 64 for the
+    # tiles (node records and FP64 (r, w), 32 B per node; 16-byte entries)
+    inf = ctx.korobov_info()
+    alg = {"plain_fp64_grouped": 16.0 * n + 16.0 * inf["nodes"] + 8.0 * inf["entries"],
+           "plain_fp64_tiles": 16.0 * n + (32.0 * inf["nodes"] + 16.0 * inf["entries"]) / 64.0,
+           "mixed_fp32_tiles": 16.0 * n + (32.0 * inf["nodes"] + 16.0 * inf["entries"]) / 64.0}
+    for name, b_ in alg.items():
+        gbs = b_ * out[name]["evals_per_s"] / 1e9
+        out[name]["roofline"] = {"bound": "hbm", "alg_bytes_per_eval": int(b_), "achieved": round(gbs, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
     for name in ("plain_fp64_grouped", "plain_fp64_tiles", "mixed_fp32_tiles"):
         e = out[name]["evals_per_s"]
         out[name]["speedup_vs_plain_fp64_single"] = round(e / single_evals, 2)
@@ -565,8 +577,12 @@ def main():
                     help="replay the steps as a hipGraph (GDT_FLAG_GRAPH) instead of launching them on the stream")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the extension legs (uneven batch, matrix cores, surrogate)")
+    ap.add_argument("--ext", default="uneven_batch,batched_matrix_mfma,surrogate,transport",
+                    help="comma-separated extension legs to run (default: all four)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the 32^3 many-cell comparison of the plain, tiled and mixed forms")
+    ap.add_argument("--no-form-surrogate", action="store_true",
+                    help="skip the surrogate (training and inference) in the 32^3 form comparison")
     ap.add_argument("--form-cells", type=int, default=2048, help="cells of the 32^3 form comparison")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
@@ -672,14 +688,17 @@ def main():
     value = args.steps / elapsed
     forms = None
     if not args.no_forms:
-        forms = forms_32cube(G, ctx, value, cells=args.form_cells)
+        forms = forms_32cube(G, ctx, value, cells=args.form_cells, surrogate=not args.no_form_surrogate)
 
     ext = {}
+    legs = {"uneven_batch": uneven_batch, "batched_matrix_mfma": batched_matrix, "surrogate": surrogate_form,
+            "transport": transport_step}
     if not args.no_batch:
-        ext["uneven_batch"] = uneven_batch(G)
-        ext["batched_matrix_mfma"] = batched_matrix(G)
-        ext["surrogate"] = surrogate_form(G)
-        ext["transport"] = transport_step(G)
+        for name in [x for x in args.ext.split(",") if x]:
+            if name not in legs:
+                print("bench.py: unknown extension leg %r (known: %s)" % (name, ", ".join(legs)), file=sys.stderr)
+                sys.exit(2)
+            ext[name] = legs[name](G)
 
     out = {
         "metric": "Q(f,f) evals/s at 32^3 grid: plain FP64 vs mixed/surrogate @ rel-err 1e-4",

@@ -291,9 +291,10 @@ class Context:
         tiles of a built stencil."""
         check(lib().gdt_korobov_set_tiles(self.h, int(tiles_max)))
 
-    def korobov_cell_tiles(self, chunk_tiles=4, node_blocks=2048, gather_blocks=2048, wide=0):
+    def korobov_cell_tiles(self, chunk_tiles=4, node_blocks=2048, gather_blocks=4096, wide=1):
         """Launch shape of the cell-tile paths (chunk of tiles, grid sizes,
-        tile width 16 / 32 or 64 cells)."""
+        tile width 16 / 32 or 64 cells); the defaults are the library's
+        own, which a new context starts with."""
         check(lib().gdt_korobov_cell_tiles(self.h, int(chunk_tiles), int(node_blocks), int(gather_blocks),
                                            int(wide)))
 
