@@ -40,11 +40,14 @@
 // Chunks of tiles bound the node values in flight between the node and
 // gather kernels (4 tiles of 64 cells: 412 MB at 32^3 and 2^21 nodes in FP32).
 //
-// What bounds them (counters at 32^3, 2^21 nodes): both kernels keep the L2
-// channels about 90% busy.  The gather reads each node's D row about 5.7
-// times (once per destination it feeds, scattered over the grid), 80% of
-// those reads miss L2 (about 690 cycles each) and the misses stream 64-byte
-// requests from the memory side at about 3.9 TB/s.
+// What bounds them (counters at 32^3, 2^21 nodes): the gather reads each
+// node's D row about 5.7 times (once per destination it feeds, scattered over
+// the grid) and each role-0 entry's partner row of f; a tile's rows (103 MB
+// in FP32) do not stay in L2, so 79% (FP32) to 92% (FP64) of its L2 requests
+// miss (about 690 cycles each), and its FETCH_SIZE + WRITE_SIZE, scaled by a
+// copy of known size, come to about 7.8 TB/s -- the memory side's rate, for
+// any chunk or grid size.  The node kernel writes each row once and reads the
+// f tile (4.4 MB in FP32, a little over one XCD's L2) at 5.4 TB/s.
 #include <algorithm>
 
 #include "gdt_internal.h"
