@@ -198,12 +198,16 @@ def batched_matrix(G, cells=8192, ref_cells=256, seed=SEED):
         err = float(np.linalg.norm(Qm - Qr) / np.linalg.norm(Qr))
         errs = np.linalg.norm(Qm - Qr, axis=1) / np.linalg.norm(Qr, axis=1)
         # matrix-core work per cell: the gain GEMMs (512 g x 512 n1 per parity
-        # class times its s columns: 31^3 over the 8 classes, 31 x 1,024 as
-        # executed, the y and z extents padded to 16) and the loss GEMM (4,096 x
-        # 4,096); the split mode issues three products per product
+        # class times its s columns: 31^3 over the 8 classes) and the loss GEMM
+        # (4,096 x 4,096).  As executed, a tile of 128 columns at s_x runs
+        # 64 (kx + 1) rows and n1 each, kx = (min(s_x, 30 - s_x) - p_x) / 2
+        # (past that T is zero and no G row is read), over two tiles per s_x;
+        # the split mode issues three products per product
         prods = 3 if mode == G.GEMM_BF16_SPLIT else 1
         alg_cell = 2 * 512 * 512 * 31 ** 3 + 2 * 4096 * 4096
-        exe_cell = prods * (2 * 512 * 512 * 31 * 1024 + 2 * 4096 * 4096)
+        exe_gain = sum(2 * 2 * (64 * ((min(sx, 30 - sx) - px) // 2 + 1)) ** 2 * 128
+                       for p in range(8) for px in [(p >> 2) & 1] for sx in range(px, 31, 2))
+        exe_cell = prods * (exe_gain + 2 * 4096 * 4096)
         peak = MFMA_PEAK_TFLOPS["fp32" if mode == G.GEMM_F32 else "bf16"]
         exe_tf = exe_cell * cells / best / 1e12
         out[name] = {"evals_per_s": round(cells / best, 1), "ms": round(1e3 * best, 2),

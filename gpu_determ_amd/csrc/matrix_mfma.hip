@@ -130,6 +130,17 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
         // tile t: sx = 2 (t / 2) + px; column c: iy = 8 (t % 2) + c / 16,
         // iz = c % 16 (off the class where i_a >= 16 - p_a: T = 0)
         const int sx_t = 2 * (t >> 1) + px;
+        // On the x axis both n (the K index n1 = 64 (n_x >> 1) + ...) and the
+        // G rows read back (row block = wave w = |g_x| >> 1) are bounded by
+        // min(sx, 30 - sx): T[n1, s] is zero for every column of the tile past
+        // n_x = min(sx, 30 - sx) (the drop rule A_x <= 15, s_x - n_x >= 0), and
+        // no pair (v, v1) on the grid has |g_x| beyond it.  So the K loop stops
+        // at kx + 1 blocks of 64 n1, and waves past kx neither load their a
+        // rows nor run MFMAs (about half the work; the slices skipped would
+        // add exact zeros, and the rows skipped are never read).
+        const int kx = (min(sx_t, 2 * (MX_N - 1) - sx_t) - px) >> 1;
+        const int kend = 64 * (kx + 1);
+        const bool live = w <= kx;
         f32x16 acc[2][4];
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -138,7 +149,7 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
 #pragma unroll
                 for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
         constexpr int KB = MODE == 2 ? MX_KF : MX_KB;
-        for (int k0 = 0; k0 < MX_K; k0 += KB) {
+        for (int k0 = 0; k0 < kend; k0 += KB) {
             __syncthreads();  // previous slice's operand reads are done
             if (MODE != 2) {
                 // a slice: 512 rows x 32 n1 = 2048 chunks of 8 bf16 per part
@@ -148,8 +159,9 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         const int q = threadIdx.x + i * MX_T, row = q / 4, col = (q % 4) * 8;
-                        *(uint4 *)(&sA[(pp * MX_G + row) * MX_PB + col]) =
-                            *(const uint4 *)(src + aoff + (size_t)row * MX_K + k0 + col);
+                        if (row < kend)
+                            *(uint4 *)(&sA[(pp * MX_G + row) * MX_PB + col]) =
+                                *(const uint4 *)(src + aoff + (size_t)row * MX_K + k0 + col);
                     }
                 }
                 // T slice: 128 s x 32 n1, 8 per thread.  Thread q holds n1 =
@@ -176,6 +188,7 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const int q = threadIdx.x + i * MX_T, row = q / 4, col = (q % 4) * 4;
+                    if (row >= kend) continue;
                     const float4 v = *(const float4 *)(Awf + aoff + (size_t)row * MX_K + k0 + col);
                     float *d = &fA[row * MX_PF + col];
                     d[0] = v.x;
@@ -194,6 +207,7 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
                 }
             }
             __syncthreads();
+            if (!live) continue;
             if (MODE != 2) {
 #pragma unroll
                 for (int kk = 0; kk < MX_KB; kk += 16) {
