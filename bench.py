@@ -118,6 +118,7 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
     # plain method; mixed: FP32 node terms), graph replay
     tiles = {}
     for name, fl in (("fp64_tiles", G.FLAG_CELL_TILES), ("mixed_fp32_tiles", G.FLAG_MIXED)):
+        tk, tg, tms = b.tune(G.FLAG_CORRECT | fl)  # launch shape of the classes' tiles
         fl = G.FLAG_CORRECT | G.FLAG_GRAPH | fl
         b.run(DT, 1, fl)  # scratch, capture, warm-up
         b.sync()
@@ -125,7 +126,8 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
         b.run(DT, steps, fl)
         b.sync()
         e = time.perf_counter() - t0
-        tiles[name] = {"evals_per_s": round(cells * steps / e, 1), "ms_per_step": round(1e3 * e / steps, 3)}
+        tiles[name] = {"evals_per_s": round(cells * steps / e, 1), "ms_per_step": round(1e3 * e / steps, 3),
+                       "tuned": {"cells_per_chunk": tk, "gather_blocks": tg, "best_eval_ms": round(tms, 3)}}
     # roofline per the algorithmic bytes of one evaluation: f in + Q out (16 N)
     # plus the class stencil over the cells sharing one pass of it -- one
     # cell for the work lists (16-byte node records, 8-byte entries), 64

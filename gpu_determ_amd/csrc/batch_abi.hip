@@ -365,6 +365,36 @@ int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node
         }
         return GDT_OK;
     };
+    if (batch_tiles(flags)) {
+        // cell tiles: the classes' launch shape -- tiles whose node terms are
+        // in flight, node and gather grids -- set on every class as
+        // gdt_korobov_cell_tiles does; the tile width stays (it sets the
+        // order of the moment sums), so results do not change
+        drop_graph(b);
+        float best = 1e30f;
+        int bc = 4, bn = 2048, bg = 4096;
+        for (int chunk : {2, 4, 8})
+            for (int nb : {1024, 2048})
+                for (int gb : {2048, 4096}) {
+                    for (gdt_context *c : b->cls)
+                        if (int rc = gdt_korobov_cell_tiles(c, chunk, nb, gb, c->ct_wide)) return rc;
+                    if (int rc = batch_tiles_prepare(b, flags)) return rc;
+                    float t;
+                    if (int rc = time_step(t)) return rc;
+                    if (t < best) {
+                        best = t;
+                        bc = chunk;
+                        bn = nb;
+                        bg = gb;
+                    }
+                }
+        for (gdt_context *c : b->cls)
+            if (int rc = gdt_korobov_cell_tiles(c, bc, bn, bg, c->ct_wide)) return rc;
+        if (seg_cells) *seg_cells = bc * ct_cells_per_tile(b->cls[0], (flags & GDT_FLAG_MIXED) != 0u);
+        if (node_q) *node_q = bg;
+        if (best_ms) *best_ms = best;
+        return GDT_OK;
+    }
     float best_t = 1e30f;
     int bt = KOR_TILES_MAX;
     for (int tl : {32, 64, 128}) {

@@ -38,7 +38,8 @@
 // No atomics: every sum has a fixed order, so results are reproducible run to
 // run and do not depend on the launch shape or on the other cells of a tile.
 // Chunks of tiles bound the node values in flight between the node and
-// gather kernels (4 tiles of 64 cells: 412 MB at 32^3 and 2^21 nodes in FP32).
+// gather kernels (8 tiles of 64 cells by default: 825 MB at 32^3 and 2^21
+// nodes in FP32; the batch autotuner picks 2, 4 or 8 per batch).
 //
 // What bounds them (counters at 32^3, 2^21 nodes): the gather reads each
 // node's D row about 5.7 times (once per destination it feeds, scattered over

@@ -295,7 +295,12 @@ int gdt_batch_config(gdt_batch_handle b, int seg_cells, int node_q);
  * classes' gather tile limit (32, 64, 128; set on every class with
  * gdt_korobov_set_tiles, so single runs of those contexts keep matching the
  * batch bitwise), then every (segment size, nodes per thread) candidate, and
- * keeps the fastest. */
+ * keeps the fastest.  With GDT_FLAG_CELL_TILES or GDT_FLAG_MIXED in flags it
+ * tunes the cell tiles instead: chunk of tiles (2, 4, 8) x node grid (1024,
+ * 2048) x gather grid (2048, 4096), set on every class as
+ * gdt_korobov_cell_tiles does with the classes' own tile width (results do
+ * not change); *seg_cells then returns the cells per chunk and *node_q the
+ * gather grid. */
 int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node_q, double *best_ms);
 int gdt_batch_run(gdt_batch_handle b, double dt, int steps, unsigned flags);
 int gdt_batch_synchronize(gdt_batch_handle b);

@@ -222,3 +222,40 @@ def test_batch_cell_tiles_mode():
             assert np.array_equal(u, v.get())
         b.close()
         b2.close()
+
+
+def test_batch_tune_cell_tiles():
+    """The autotuner in cell-tile mode picks a launch shape from its candidates
+    (cells per chunk 2, 4 or 8 tiles of 64; gather grid 2048 or 4096), leaves
+    f unchanged, and the results after tuning equal those before, bit for
+    bit, eager and as graph replay."""
+    print("batch tune, cell tiles", flush=True)
+    classes = make_classes((16, 20), M=1 << 16, seed=3)
+    rng = np.random.default_rng(23)
+    cells = [150, 9]
+    f0 = [cell_fs(c, k, rng) for c, k in zip(classes, cells)]
+    for fl in (G.FLAG_CELL_TILES, G.FLAG_MIXED):
+        fd = [G.DeviceArray.from_host(x) for x in f0]
+        qd = [G.DeviceArray((k, c.n_pnt)) for c, k in zip(classes, cells)]
+        b = G.Batch(classes, fd, qd)
+        b.run(0.0, 1, G.FLAG_CORRECT | fl)
+        b.sync()
+        before = [q.get() for q in qd]
+        k, gb, ms = b.tune(G.FLAG_CORRECT | fl)
+        print("  flags %d: cells per chunk %d, gather grid %d, %.3f ms" % (fl, k, gb, ms), flush=True)
+        assert k in (128, 256, 512) and gb in (2048, 4096) and ms > 0.0
+        for x0, d in zip(f0, fd):
+            assert np.array_equal(d.get(), x0)
+        for q in qd:
+            q.zero()
+        b.run(0.0, 1, G.FLAG_CORRECT | fl | G.FLAG_GRAPH)
+        b.sync()
+        for q, q0 in zip(qd, before):
+            assert np.array_equal(q.get(), q0)
+        for q in qd:
+            q.zero()
+        b.run(0.0, 1, G.FLAG_CORRECT | fl)
+        b.sync()
+        for q, q0 in zip(qd, before):
+            assert np.array_equal(q.get(), q0)
+        b.close()
