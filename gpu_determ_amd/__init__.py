@@ -431,6 +431,10 @@ class Batch:
         check(lib().gdt_batch_config(self.h, int(seg_cells), int(node_q)))
 
     def tune(self, flags=FLAG_CORRECT):
+        """Autotune on the batch's cells: (cells per segment, nodes per thread,
+        best evaluation ms) for the work lists; with FLAG_CELL_TILES or
+        FLAG_MIXED the classes' cell-tile shape, returned as (cells per chunk,
+        gather grid, best evaluation ms)."""
         k, q, ms = C.c_int(), C.c_int(), C.c_double()
         check(lib().gdt_batch_tune(self.h, flags, C.byref(k), C.byref(q), C.byref(ms)))
         return k.value, q.value, ms.value
