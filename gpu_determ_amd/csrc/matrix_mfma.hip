@@ -124,8 +124,19 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
 #pragma unroll
     for (int k = 0; k < 8; k++) gacc[k] = 0.0;
     // the iz of the outputs' partners v1 = s - v on the grid: sz - vz in
-    // [0, 15], i.e. iz in [izlo, izlo + 7] (at most 8 values of step 2)
+    // [0, 15], i.e. iz in [izlo, izlo + 7] (at most 8 values of step 2); per
+    // m the LDS offset of G[gz, iz] in a staged row and whether it is on the
+    // grid (off-grid m read a valid slot and add an exact zero, so the eight
+    // reads of a row go out together with no branch between them)
     const int izlo = (vz - pz + 1) >> 1;
+    int zoff[8];
+    bool zon[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int iz = izlo + m, sz = 2 * iz + pz;
+        zon[m] = iz < MX_N - pz && sz - vz <= 15;
+        zoff[m] = zon[m] ? (abs(2 * vz - sz) >> 1) * MX_GP + iz : 0;
+    }
     for (int t = 0; t < NT; t++) {
         // tile t: sx = 2 (t / 2) + px; column c: iy = 8 (t % 2) + c / 16,
         // iz = c % 16 (off the class where i_a >= 16 - p_a: T = 0)
@@ -266,22 +277,21 @@ __global__ __launch_bounds__(MX_T) void k_mx_gain(const uint16_t *__restrict__ A
 #pragma unroll
             for (int a = 0; a < 2; a++) {
                 const int iy = 8 * (t & 1) + 2 * j + a;
-                if (iy >= MX_N - py) continue;
                 const int sy = 2 * iy + py;
-                if ((unsigned)(sy - vy) > 15u) continue;
-                const int gy = abs(2 * vy - sy) >> 1;
+                // the row is on the grid and holds partners of this lane's vy
+                const bool yon = iy < MX_N - py && (unsigned)(sy - vy) <= 15u;
+                const int gy = yon ? abs(2 * vy - sy) >> 1 : 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
                     const int vx = 2 * k + (w >> 2);  // the same in the wave
                     if ((unsigned)(sx - vx) > 15u) continue;
                     const int base = (64 * (abs(2 * vx - sx) >> 1) + 8 * gy) * MX_GP + 16 * a;
+                    float gv[8];
+#pragma unroll
+                    for (int m = 0; m < 8; m++) gv[m] = Gc[base + zoff[m]];
                     double sum = gacc[k];
 #pragma unroll
-                    for (int m = 0; m < 8; m++) {
-                        const int iz = izlo + m, sz = 2 * iz + pz;
-                        if (iz < MX_N - pz && sz - vz <= 15)
-                            sum += (double)Gc[base + (abs(2 * vz - sz) >> 1) * MX_GP + iz];
-                    }
+                    for (int m = 0; m < 8; m++) sum += (yon && zon[m]) ? (double)gv[m] : 0.0;
                     gacc[k] = sum;
                 }
             }
