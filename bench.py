@@ -279,6 +279,7 @@ def train_surrogates(G, ctx, rng, train_cells, epochs, hidden, lr, seed=SEED):
         s = G.Surrogate(ctx, hidden=hidden, seed=seed + 101 * k)
         losses.append(s.train(ftr, Qtr, epochs=epochs, batch=256, lr=lr))
         nets.append(s)
+        progress("surrogate network %d of 2 trained (%d pairs, %d epochs)" % (k + 1, train_cells, epochs))
     train_s = time.perf_counter() - t0
     nets[0].set_partner(nets[1])
     info = {"train_pairs": train_cells, "epochs": epochs, "hidden": hidden, "lr": lr, "params_per_network": nets[0].params,
@@ -569,6 +570,11 @@ def cpu_baseline(xi, f0, n_pnt, d3v, budget_s=15.0):
                       % "/".join(str(v[1]) for v in res.values())}
 
 
+def progress(msg):
+    """One line on stderr per finished leg, so a long default run shows where it is."""
+    print("bench.py: %s" % msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -626,6 +632,7 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     f_end = fd.get()[0]
+    progress("headline steps done (%.1f evals/s)" % (args.steps / elapsed))
 
     m0 = moments(ctx.xi, f_start, ctx.d3v)
     m1 = moments(ctx.xi, f_end, ctx.d3v)
@@ -694,11 +701,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ctx.xi, f0, n, ctx.d3v)
+        progress("cpu baseline done")
 
     value = args.steps / elapsed
     forms = None
     if not args.no_forms:
         forms = forms_32cube(G, ctx, value, cells=args.form_cells, surrogate=not args.no_form_surrogate)
+        progress("32^3 form comparison done")
 
     ext = {}
     legs = {"uneven_batch": uneven_batch, "batched_matrix_mfma": batched_matrix, "surrogate": surrogate_form,
@@ -709,6 +718,7 @@ def main():
                 print("bench.py: unknown extension leg %r (known: %s)" % (name, ", ".join(legs)), file=sys.stderr)
                 sys.exit(2)
             ext[name] = legs[name](G)
+            progress("extension leg %s done" % name)
 
     out = {
         "metric": "Q(f,f) evals/s at 32^3 grid: plain FP64 vs mixed/surrogate @ rel-err 1e-4",
