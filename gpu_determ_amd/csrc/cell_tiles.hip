@@ -18,17 +18,20 @@
 // Per evaluation of C cells in T = ceil(C / K) tiles (cells past C are zero):
 //   k_ct_pack     f [cell][node] FP64  ->  ft[tile][node][K] of T
 //   k_ct_node     per chunk of tiles: D of every (node, cell) into
-//                 D[tile][p][K] (p: the sorted node position); r and w come
-//                 precomputed per node
-//   k_ct_gather   per chunk: the K lanes of a destination walk its entries of
-//                 the global destination-sorted CSR in order (one FP64 sum per
-//                 cell, no partials over node groups): Q0[tile][d][K] = sum
-//                 coef D and the loss L[tile][d][K] = f_d sum 2 w f_partner
-//                 over the entries where d is alpha or beta (the loss is
-//                 linear in f given f_d, so it reads the f tile instead of a
-//                 stored 2 w P: half the node kernel's writes and a quarter of
-//                 the gather's reads); destinations are taken longest first,
-//                 so concurrent waves have similar entry counts
+//                 D[tile][q][K] (q: the node's position in the tile order,
+//                 sorted by the alpha destination); r and w come precomputed
+//                 per node.  A destination's alpha nodes are one run of
+//                 positions, walked by one lane group, which also sums the
+//                 run's alpha terms (-D into Q0, 2 w f_beta into the loss)
+//   k_ct_gather   per chunk: the K lanes of a destination start from those
+//                 sums and walk its entries of the destination-sorted tile CSR
+//                 (beta, lambda, mu, lambda+s, mu-s) in order (one FP64 sum
+//                 per cell, no partials over node groups): Q0[tile][d][K] =
+//                 sum coef D and the loss L[tile][d][K] = f_d sum 2 w
+//                 f_partner over the entries where d is alpha or beta (the
+//                 loss is linear in f given f_d, so it reads the f tile
+//                 instead of a stored 2 w P); destinations are taken longest
+//                 first, so concurrent waves have similar entry counts
 //   k_ct_moments  per (destination range, tile): the 20 moment sums of the
 //                 range, one partial per cell and range
 //   k_ct_solve    per cell: the partials in a fixed order, the 5x5 solve: X
@@ -109,27 +112,40 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_pack(int npts, int cells, con
 // indices, and num = 0 where r = 0.
 __device__ __forceinline__ int ct_num(uint32_t w) { return (int)(w & 0x7FFu); }
 
+// The six f values of a node for one lane: alpha's (a, the same for a whole
+// alpha run, so the caller passes it in), beta's, and the projection pairs'.
+template <class T> struct CtF {
+    T a, b, l, m, ls, ms;
+};
+template <class T, class LD>
+__device__ __forceinline__ CtF<T> ct_load(const uint4 x, T fa, LD ld)
+{
+    CtF<T> v;
+    v.a = fa;
+    v.b = ld((x.x >> 15) & 0x7FFFu);
+    v.l = ld(x.y & 0x7FFFu);
+    v.m = ld(x.y >> 15);
+    v.ls = ld(x.z & 0x7FFFu);
+    v.ms = ld(x.z >> 15);
+    return v;
+}
+
 // FP64: the operations of the single-cell node kernel (table-driven log and
-// exp from ts, contraction off), so D is the same bits.  ld(i) is the lane's
-// f value of grid node i.
+// exp from ts, contraction off), so D is the same bits.
 // r = num / den and w = K sqrt(D2) come precomputed per node (rw: the same
 // correctly rounded values the single-cell kernel forms).
-template <class LD>
-__device__ __forceinline__ double ct_terms(const uint4 x, const double2 rw, LD ld, const double *ts)
+__device__ __forceinline__ double ct_terms(const uint4 x, const double2 rw, const CtF<double> &v, const double *ts)
 {
 #pragma clang fp contract(off)
     const int num = ct_num(x.w);
-    const double fa = ld(x.x & 0x7FFFu), fb = ld((x.x >> 15) & 0x7FFFu);
-    const double fl = ld(x.y & 0x7FFFu), fm = ld(x.y >> 15);
-    const double fls = ld(x.z & 0x7FFFu), fms = ld(x.z >> 15);
-    const double P = fa * fb;
-    const double Pl = fl * fm;
+    const double P = v.a * v.b;
+    const double Pl = v.l * v.m;
     double R;
     if (num == 0) {
         R = Pl;
     } else {
         const double r = rw.x;
-        const double Ps = fls * fms;
+        const double Ps = v.ls * v.ms;
         if (Pl > 0.0 && Ps > 0.0)
             R = Pl * kmt_exp(r * kmt_log(Ps / Pl, ts), ts);
         else
@@ -142,20 +158,16 @@ __device__ __forceinline__ double ct_terms(const uint4 x, const double2 rw, LD l
 // transcendentals.  The ratio Ps / Pl is split by frexp into mantissas (whose
 // quotient lies in (1/2, 2): v_log_f32 never sees a denormal) and an exact
 // integer exponent difference; the quotient is one reciprocal refined by a
-// Newton step (within an ulp of the rounded quotient); r = num / den by the
-// r and w come per node, rounded to FP32.  R = Pl 2^(r log2(Ps / Pl)); an
-// exponent below -126 gives 0 (a value below 2^-126 Pl, far below the sums'
-// rounding).  Products that are not positive take the linear form, as in FP64.
-template <class LD>
-__device__ __forceinline__ float ct_terms(const uint4 x, const float2 rw, LD ld, const double *ts)
+// Newton step (within an ulp of the rounded quotient); r and w come per
+// node, rounded to FP32.  R = Pl 2^(r log2(Ps / Pl)); an exponent below -126
+// gives 0 (a value below 2^-126 Pl, far below the sums' rounding).  Products
+// that are not positive take the linear form, as in FP64.
+__device__ __forceinline__ float ct_terms(const uint4 x, const float2 rw, const CtF<float> &v, const double *ts)
 {
     (void)ts;
     const int num = ct_num(x.w);
-    const float fa = ld(x.x & 0x7FFFu), fb = ld((x.x >> 15) & 0x7FFFu);
-    const float fl = ld(x.y & 0x7FFFu), fm = ld(x.y >> 15);
-    const float fls = ld(x.z & 0x7FFFu), fms = ld(x.z >> 15);
-    const float P = fa * fb;
-    const float Pl = fl * fm, Ps = fls * fms;
+    const float P = v.a * v.b;
+    const float Pl = v.l * v.m, Ps = v.ls * v.ms;
     const float r = rw.x;
     float R = Pl;
     if (num != 0) {
@@ -172,6 +184,16 @@ __device__ __forceinline__ float ct_terms(const uint4 x, const float2 rw, LD ld,
         }
     }
     return rw.y * (P - R);
+}
+
+// One term of a destination's FP64 sum q: the FP64 tiles keep the
+// single-cell path's term (coef D, then the sum), the mixed form one FMA.
+template <class T>
+__device__ __forceinline__ double ct_acc(double q, double coef, T d)
+{
+#pragma clang fp contract(off)
+    if (sizeof(T) == 8) return q + coef * (double)d;
+    return fma(coef, (double)d, q);
 }
 
 // Lane group of K lanes: its group index, made uniform (an SGPR) when the
@@ -194,12 +216,18 @@ __device__ __forceinline__ T ct_row(const T *__restrict__ fr, uint32_t i, int k)
     return ld_off(fr, i * K + k);
 }
 
-// Block (x, j) of a chunk: tile t0 + j; lane group g = tid / K takes nodes
-// p = x NPB + g, + gridDim.x NPB, ... (NPB = CT_THREADS / K per block and
-// pass); lane k of the group is cell k of the tile.  The record load is one
+// Block (x, j) of a chunk: tile t0 + j; lane group g = tid / K takes the
+// destination slots s = x NPB + g, + gridDim.x NPB, ... (NPB = CT_THREADS / K)
+// of the longest-first order aperm; lane k of the group is cell k of the
+// tile.  For destination a the group walks a's alpha run, positions
+// [aptr[a], aptr[a + 1]) of the tile node order, two nodes per pass (their
+// loads in flight together): it stores each node's D row, and sums the run's
+// alpha terms of Q0 (coefficient -1) and of the loss (2 w f_beta, FP64 w)
+// in the CSR's term form; the sums start the gather's sums of destination a
+// (Q0 and L of the tile, L before its factor f_a).  Record loads are one
 // address per group; the f loads and the value stores are K contiguous
 // elements (a 64- or 128-byte row).
-// Dv holds the chunk's D values [j][p][K].
+// Dv holds the chunk's D values [j][q][K].
 template <class T> struct CtRW;
 template <> struct CtRW<double> {
     typedef double2 type;
@@ -208,49 +236,67 @@ template <> struct CtRW<float> {
     typedef float2 type;
 };
 
-// Rows hold RS cells; lane groups of K = RS cells.  (Slices of K < RS cells per
-// launch keep a slice of the f tile in L2, but lose the wave-uniform records:
-// 0.90 and 0.98 us per cell for halves and quarters against 0.84 us.  Slices
-// per XCD in one launch -- block b on cells (b % 8) % 2 of every row, so each
-// XCD reads a 2.2 MB half of the f tile -- measured 0.85 against 0.77 us in
-// FP32 and 1.86 against 1.69 in FP64 quarters, with the same bits.)
-template <class T, int K, int RS>
-__global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const uint4 *__restrict__ nrec,
+// (Slices of K < 64 cells per launch keep a slice of the f tile in L2, but
+// lose the wave-uniform records: 0.90 and 0.98 us per cell for halves and
+// quarters against 0.84 us.  Slices per XCD in one launch -- block b on cells
+// (b % 8) % 2 of every row, so each XCD reads a 2.2 MB half of the f tile --
+// measured 0.85 against 0.77 us in FP32 and 1.86 against 1.69 in FP64
+// quarters, with the same bits.)
+template <class T, int K>
+__global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const int32_t *__restrict__ aptr,
+                                                        const int32_t *__restrict__ aperm,
+                                                        const uint4 *__restrict__ nrec,
                                                         const typename CtRW<T>::type *__restrict__ nrw,
+                                                        const double2 *__restrict__ rw64,
                                                         const T *__restrict__ ft, int t0, T *__restrict__ Dv,
+                                                        double *__restrict__ Q0, double *__restrict__ L,
                                                         const double *__restrict__ kmt)
 {
+#pragma clang fp contract(off)
     __shared__ double ts[KMT_SIZE];
     if (sizeof(T) == 8)
         for (int i = threadIdx.x; i < KMT_SIZE; i += CT_THREADS) ts[i] = kmt[i];
     __syncthreads();
     constexpr int NPB = CT_THREADS / K;
     const int k = threadIdx.x % K, g = ct_group<K>();
-    const T *fr = ft + (size_t)(t0 + blockIdx.y) * npts * RS;
-    T *Dt = Dv + (size_t)blockIdx.y * nn * RS;
-    auto ld = [&](uint32_t i) {
-        if (K == WAVE && K == RS) return (fr + (size_t)i * RS)[k];
-        return ld_off(fr, i * RS + k);
-    };
-    // two nodes per pass (p and p + stride), evaluated together so that their
-    // loads are in flight at once; streaming stores for the wide rows, so
-    // the rows do not push the f tile out of L2 (node kernel 0.805 -> 0.783
-    // us per cell in FP32, 1.764 -> 1.655 in FP64; the gather is unchanged,
-    // and streaming loads there are slower: 1.62 -> 1.89)
-    auto store = [&](int p, T d) {
-        if (K == WAVE && K == RS)
-            __builtin_nontemporal_store(d, &(Dt + (size_t)p * RS)[k]);
+    const T *fr = ft + (size_t)(t0 + blockIdx.y) * npts * K;
+    T *Dt = Dv + (size_t)blockIdx.y * nn * K;
+    const size_t to = (size_t)(t0 + blockIdx.y) * npts * K;
+    auto ld = [&](uint32_t i) { return ct_row<K>(fr, i, k); };
+    // streaming stores for the wide rows, so the rows do not push the f tile
+    // out of L2 (node kernel 0.805 -> 0.783 us per cell in FP32, 1.764 ->
+    // 1.655 in FP64; the gather is unchanged, and streaming loads there are
+    // slower: 1.62 -> 1.89)
+    auto store = [&](int q, T d) {
+        if (K == WAVE)
+            __builtin_nontemporal_store(d, &(Dt + (size_t)q * K)[k]);
         else
-            *(T *)((char *)Dt + (size_t)(uint32_t)(((uint32_t)p * RS + k) * (uint32_t)sizeof(T))) = d;
+            *(T *)((char *)Dt + (size_t)(uint32_t)(((uint32_t)q * K + k) * (uint32_t)sizeof(T))) = d;
     };
-    const int stride = gridDim.x * NPB;
-    for (int p = blockIdx.x * NPB + g; p < nn; p += 2 * stride) {
-        const int p2 = p + stride < nn ? p + stride : p;  // clamped: in bounds
-        const uint4 x = nrec[p], x2 = nrec[p2];
-        const T d = ct_terms(x, nrw[p], ld, ts);
-        const T d2 = ct_terms(x2, nrw[p2], ld, ts);
-        store(p, d);
-        if (p + stride < nn) store(p2, d2);
+    for (int s = blockIdx.x * NPB + g; s < npts; s += gridDim.x * NPB) {
+        const int a = aperm[s];
+        const int q0 = aptr[a], q1 = aptr[a + 1];
+        const T fa = ld((uint32_t)a);
+        double qs = 0.0, ls = 0.0;
+        for (int q = q0; q < q1; q += 2) {
+            const bool two = q + 1 < q1;
+            const int qb = two ? q + 1 : q;  // clamped: in bounds
+            const uint4 x = nrec[q], x2 = nrec[qb];
+            const CtF<T> v = ct_load(x, fa, ld), v2 = ct_load(x2, fa, ld);
+            const double w1 = rw64[q].y, w2 = rw64[qb].y;
+            const T d = ct_terms(x, nrw[q], v, ts);
+            const T d2 = ct_terms(x2, nrw[qb], v2, ts);
+            store(q, d);
+            qs = ct_acc(qs, -1.0, d);
+            ls += (2.0 * w1) * (double)v.b;
+            if (two) {
+                store(qb, d2);
+                qs = ct_acc(qs, -1.0, d2);
+                ls += (2.0 * w2) * (double)v2.b;
+            }
+        }
+        Q0[to + (size_t)a * K + k] = qs;
+        L[to + (size_t)a * K + k] = ls;
     }
 }
 
@@ -259,10 +305,11 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const 
 // ---------------------------------------------------------------------------
 // Lane group g of block (x, j) takes destination slots s = x NPB + g, +
 // gridDim.x NPB, ... of the longest-first order dperm; lane k sums cell k of
-// the destination in the CSR's order: q = sum coef D (coef -1 for role 0) and
-// l = sum 2 w f_partner over the role-0 entries, then L = f_d l.  FP64
-// products and sums; the FP64 tiles keep the single-cell path's term for q
-// (coef D, then the sum), the mixed form one FMA per term.  Each round issues
+// the destination: q and l start from the node kernel's sums over d's alpha
+// run, then add the tile CSR's entries in order: q += coef D (coef -1 for
+// role 0, beta) and l += 2 w f_alpha over the role-0 entries; L = f_d l.
+// FP64 products and sums; the FP64 tiles keep the single-cell path's term
+// for q (coef D, then the sum), the mixed form one FMA per term.  Each round issues
 // the loads of CT_UNROLL entries (records, then D values and f values) before
 // adding any of them, with no branch between the loads: every entry loads a D
 // value and an f value (its partner's for role 0, the destination's own --
@@ -288,7 +335,8 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, cons
     for (int s = blockIdx.x * NPB + g; s < npts; s += gridDim.x * NPB) {
         const int d = dperm[s];
         const int e0 = cptr[d], e1 = cptr[d + 1];
-        double q = 0.0, l = 0.0;
+        // the node kernel's sums of d's alpha run come first
+        double q = Q0[to + (size_t)d * K + k], l = L[to + (size_t)d * K + k];
         for (int e = e0; e < e1; e += CT_UNROLL) {
             uint4 x[CT_UNROLL];
 #pragma unroll
@@ -316,10 +364,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, cons
                 const bool in = e + u < e1, role0 = (x[u].x & 3u) == 0u;
                 const double val = __hiloint2double((int)x[u].w, (int)x[u].z);
                 const double coef = in ? (role0 ? -1.0 : val) : 0.0;
-                if (sizeof(T) == 8)
-                    q += coef * (double)dv[u];  // the single-cell path's term, then the sum
-                else
-                    q = fma(coef, (double)dv[u], q);  // mixed: one rounding per term
+                q = ct_acc(q, coef, dv[u]);
                 l += (in && role0) ? val * (double)fv[u] : 0.0;
             }
         }
@@ -484,8 +529,9 @@ static int ct_launch(gdt_context *h, const double *f, int cells, bool correct, d
     for (int t0 = 0; t0 < tiles; t0 += chunk) {
         const int nt = std::min(chunk, tiles - t0);
         const auto *nrw = (const typename CtRW<T>::type *)(sizeof(T) == 8 ? h->krw.p : h->krwf.p);
-        GDT_LAUNCH_TIMED(h, "ct_node", (k_ct_node<T, K, K>), dim3(nblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
-                         h->kn4.as<uint4>(), nrw, (const T *)ft, t0, Dv, h->kmath.as<double>());
+        GDT_LAUNCH_TIMED(h, "ct_node", (k_ct_node<T, K>), dim3(nblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
+                         h->kaptr.as<int32_t>(), h->kaperm.as<int32_t>(), h->kt4.as<uint4>(), nrw,
+                         h->krw.as<double2>(), (const T *)ft, t0, Dv, Q0, L, h->kmath.as<double>());
         GDT_LAUNCH_TIMED(h, "ct_gather", (k_ct_gather<T, K>), dim3(gblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
                          h->kcptr.as<int32_t>(), h->kcent.as<uint4>(), h->kdperm.as<int32_t>(), (const T *)Dv,
                          (const T *)ft, t0, Q0, L);
@@ -521,7 +567,8 @@ int ct_config(gdt_context *h, int chunk_tiles, int node_blocks, int gather_block
 int ct_run(gdt_context *h, bool mixed, const double *f, int cells, bool correct, double *Qout, double *Lout,
            double *nu_out, double dt, double *f_upd)
 {
-    GDT_CHECK(h->kcent.p && h->kcptr.p && h->kdperm.p && h->krw.p && h->krwf.p, GDT_E_STATE,
+    GDT_CHECK(h->kcent.p && h->kcptr.p && h->kdperm.p && h->krw.p && h->krwf.p && h->kt4.p && h->kaptr.p &&
+                  h->kaperm.p, GDT_E_STATE,
               "cell tiles: stencil not built");
     if (h->ct_wide)
         return mixed ? ct_launch<float, WAVE>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd)

@@ -693,72 +693,104 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 gent[2 * (size_t)pos + 1] = n4[4 * (size_t)p + 3];
             }
         }
-        // one destination-sorted CSR over the sorted positions (the
-        // cell-tile gather): a destination's entries follow (p, slot), i.e.
-        // its group CSRs' entries in group order
+        // the cell-tile path's own node order: sorted by the alpha
+        // destination (then lambda, then lambda+s, lattice order among
+        // equals), so that every destination's alpha entries are one
+        // contiguous run of positions q.  The tile node kernel walks a
+        // destination's run, stores each node's D row and sums the run's
+        // alpha terms (-D, and 2 w f_beta for the loss) itself; the tile CSR
+        // holds the other entries (beta, lambda, mu, lambda+s, mu-s).
         {
+            std::vector<uint32_t> tord(nn);
+            for (long t = 0; t < nn; t++) tord[t] = (uint32_t)t;
+            std::stable_sort(tord.begin(), tord.end(), [&](uint32_t x, uint32_t y) {
+                const int32_t *ox = &cidx6[6 * (size_t)x], *oy = &cidx6[6 * (size_t)y];
+                if (ox[0] != oy[0]) return ox[0] < oy[0];
+                if (ox[2] != oy[2]) return ox[2] < oy[2];
+                return ox[4] < oy[4];
+            });
+            // alpha runs: aptr[v] .. aptr[v + 1] are the positions whose alpha is v
+            std::vector<int32_t> aptr(np + 1, 0);
+            for (long q = 0; q < nn; q++) aptr[cidx6[6 * (size_t)tord[q]] + 1]++;
+            for (int v = 0; v < np; v++) aptr[v + 1] += aptr[v];
+            std::vector<uint32_t> t4(4 * (size_t)nn);
+            for (long q = 0; q < nn; q++) pack(tord[q], &t4[4 * q]);
+            // entry slots of the tile CSR: 1 (beta) and, where r != 0, 2..5;
+            // where r = 0, 1..3
             std::vector<int32_t> cp(np + 1, 0);
-            for (long p = 0; p < nn; p++) {
-                const long t = order[p];
-                for (int sl = 0; sl < slots(t); sl++) cp[cidx6[6 * t + sl] + 1]++;
+            for (long q = 0; q < nn; q++) {
+                const long t = tord[q];
+                for (int sl = 1; sl < slots(t); sl++) cp[cidx6[6 * t + sl] + 1]++;
             }
             for (int v = 0; v < np; v++) cp[v + 1] += cp[v];
-            // entry record (p << 2 | role, g, value as FP64 bits):
-            //   role 0 (alpha, beta):  g = the partner node (beta, alpha),
-            //                          value = 2 w (the loss is f_d sum 2 w f_g);
-            //   role 1 / 2:            g = the destination itself, value =
-            //                          1 - r / r, r = (num / 4) / (den / 4) --
-            //                          the quotient the group gather forms
-            std::vector<uint32_t> cent(4 * (size_t)ne);
+            const long nte = cp[np];
+            // entry record (q << 2 | role, g, value as FP64 bits):
+            //   role 0 (beta):  g = the partner node alpha, value = 2 w (the
+            //                   loss is f_d sum 2 w f_g);
+            //   role 1 / 2:     g = the destination itself, value = 1 - r / r,
+            //                   r = (num / 4) / (den / 4) -- the quotient the
+            //                   group gather forms
+            std::vector<uint32_t> cent(4 * (size_t)nte);
             std::vector<int32_t> cf(cp.begin(), cp.end() - 1);
-            for (long p = 0; p < nn; p++) {
-                const long t = order[p];
-                const uint32_t y = n4[4 * (size_t)p + 3];
+            for (long q = 0; q < nn; q++) {
+                const long t = tord[q];
+                const uint32_t y = t4[4 * (size_t)q + 3];
                 const double num4 = (double)(y & 0x7FFu), den4 = (double)((y >> 11) & 0x7FFu);
                 const double r = num4 == 0.0 ? 0.0 : num4 / den4;
-                for (int sl = 0; sl < slots(t); sl++) {
+                for (int sl = 1; sl < slots(t); sl++) {
                     const uint32_t role = sl < 2 ? 0u : (sl < 4 ? 1u : 2u);
                     const int32_t dst = cidx6[6 * t + sl];
                     const double val = role == 0u ? 2.0 * cw[t] : (role == 1u ? 1.0 - r : r);
-                    const int32_t g = role == 0u ? cidx6[6 * t + (1 - sl)] : dst;
+                    const int32_t g = role == 0u ? cidx6[6 * t] : dst;
                     uint64_t cb;
                     std::memcpy(&cb, &val, sizeof cb);
                     const int32_t pos = cf[dst]++;
-                    cent[4 * (size_t)pos] = ((uint32_t)p << 2) | role;
+                    cent[4 * (size_t)pos] = ((uint32_t)q << 2) | role;
                     cent[4 * (size_t)pos + 1] = (uint32_t)g;
                     cent[4 * (size_t)pos + 2] = (uint32_t)cb;
                     cent[4 * (size_t)pos + 3] = (uint32_t)(cb >> 32);
                 }
             }
-            // per sorted position: r and w of the node (FP64, and rounded to
+            // per tile position: r and w of the node (FP64, and rounded to
             // FP32 for the mixed node terms) -- r the quotient num / den the
             // node kernels form, w = K sqrt(D2) as the lattice kernel formed it
             std::vector<double> rw(2 * (size_t)nn);
             std::vector<float> rwf(2 * (size_t)nn);
-            for (long p = 0; p < nn; p++) {
-                const long t = order[p];
-                const uint32_t y = n4[4 * (size_t)p + 3];
+            for (long q = 0; q < nn; q++) {
+                const long t = tord[q];
+                const uint32_t y = t4[4 * (size_t)q + 3];
                 const double num4 = (double)(y & 0x7FFu), den4 = (double)((y >> 11) & 0x7FFu);
-                rw[2 * p] = num4 == 0.0 ? 0.0 : num4 / den4;
-                rw[2 * p + 1] = cw[t];
-                rwf[2 * p] = (float)rw[2 * p];
-                rwf[2 * p + 1] = (float)rw[2 * p + 1];
+                rw[2 * q] = num4 == 0.0 ? 0.0 : num4 / den4;
+                rw[2 * q + 1] = cw[t];
+                rwf[2 * q] = (float)rw[2 * q];
+                rwf[2 * q + 1] = (float)rw[2 * q + 1];
             }
-            GDT_CHECK(h->krw.ensure(sizeof(double) * 2 * nn) == 0 && h->krwf.ensure(sizeof(float) * 2 * nn) == 0,
-                      GDT_E_NOMEM, "gdt_korobov_build: node weight allocation failed");
-            GDT_HIP(hipMemcpy(h->krw.p, rw.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
-            GDT_HIP(hipMemcpy(h->krwf.p, rwf.data(), sizeof(float) * 2 * nn, hipMemcpyHostToDevice));
-            std::vector<int32_t> dperm(np);
-            for (int v = 0; v < np; v++) dperm[v] = v;
+            // destinations longest first: by tile-CSR entries (gather) and by
+            // alpha-run length (node kernel)
+            std::vector<int32_t> dperm(np), aperm(np);
+            for (int v = 0; v < np; v++) dperm[v] = aperm[v] = v;
             std::stable_sort(dperm.begin(), dperm.end(),
                              [&](int32_t x, int32_t y) { return cp[x + 1] - cp[x] > cp[y + 1] - cp[y]; });
-            GDT_CHECK(h->kcptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
-                          h->kcent.ensure(sizeof(uint32_t) * 4 * ne) == 0 &&
+            std::stable_sort(aperm.begin(), aperm.end(),
+                             [&](int32_t x, int32_t y) { return aptr[x + 1] - aptr[x] > aptr[y + 1] - aptr[y]; });
+            GDT_CHECK(h->krw.ensure(sizeof(double) * 2 * nn) == 0 && h->krwf.ensure(sizeof(float) * 2 * nn) == 0 &&
+                          h->kt4.ensure(sizeof(uint32_t) * 4 * nn) == 0 &&
+                          h->kaptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
+                          h->kaperm.ensure(sizeof(int32_t) * np) == 0 &&
+                          h->kcptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
+                          h->kcent.ensure(sizeof(uint32_t) * 4 * std::max(nte, 1L)) == 0 &&
                           h->kdperm.ensure(sizeof(int32_t) * np) == 0,
-                      GDT_E_NOMEM, "gdt_korobov_build: cell-tile CSR allocation failed");
+                      GDT_E_NOMEM, "gdt_korobov_build: cell-tile stencil allocation failed");
+            GDT_HIP(hipMemcpy(h->krw.p, rw.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->krwf.p, rwf.data(), sizeof(float) * 2 * nn, hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->kt4.p, t4.data(), sizeof(uint32_t) * 4 * nn, hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->kaptr.p, aptr.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->kaperm.p, aperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
             GDT_HIP(hipMemcpy(h->kcptr.p, cp.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice));
-            GDT_HIP(hipMemcpy(h->kcent.p, cent.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
+            if (nte > 0)
+                GDT_HIP(hipMemcpy(h->kcent.p, cent.data(), sizeof(uint32_t) * 4 * nte, hipMemcpyHostToDevice));
             GDT_HIP(hipMemcpy(h->kdperm.p, dperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+            h->ntent = nte;
         }
         h->h_gp.swap(gp);
         GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * h->h_gp.size()) == 0 &&

@@ -40,9 +40,32 @@ void gdt_set_error(int code, const char *fmt, ...);
 #define KOR_APPLY_THREADS 128 // destinations per apply block (128: 26.0 us per step; 64: 27.0, 256: 26.3, 512: 27.9)
 
 // Device buffer owned by a context.
+// Owns one device allocation: freed by release() or on destruction (so an
+// early return or a destroyed context frees every buffer); movable, not
+// copyable.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes)
+    {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int ensure(size_t need);  // grow (contents not kept)
     void release();
     template <class T> T *as() const { return static_cast<T *>(p); }
@@ -94,11 +117,16 @@ struct gdt_context {
     std::vector<int32_t> h_gp;    // host copy of the group CSR row pointers [G][n+1]
     unsigned kor_gen = 0;         // bumped whenever the stencil's gather layout changes
     std::vector<int32_t> h_kptr;  // host copy of the CSR row pointers
-    // many-cell (cell-tile) path: one destination-sorted CSR over the sorted
-    // node positions (uint4 entry records: p << 2 | role, a grid node, an FP64
-    // value), and the destinations ordered by entry count, longest first
+    // many-cell (cell-tile) path, over its own node order q (sorted by the
+    // alpha destination): packed node records, the alpha run of each
+    // destination (positions aptr[v] .. aptr[v + 1]) and the destinations by
+    // run length, longest first; one destination-sorted CSR of the other
+    // entries (uint4 records: q << 2 | role, a grid node, an FP64 value), and
+    // the destinations ordered by its entry count, longest first
+    DevBuf kt4, kaptr, kaperm;
     DevBuf kcptr, kcent, kdperm;
-    DevBuf krw, krwf;             // (r, w) per sorted node position, FP64 and FP32
+    DevBuf krw, krwf;             // (r, w) per tile node position, FP64 and FP32
+    long ntent = 0;               // entries of the tile CSR
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
     int lds_attr = 0;             // dynamic-LDS bytes the fused kernel is enabled for
     int gain_lds_attr = 0;        // dynamic-LDS bytes the matrix gain kernel is enabled for
