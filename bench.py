@@ -131,14 +131,17 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
     # roofline per the algorithmic bytes of one evaluation: f in + Q out (16 N)
     # plus the class stencil over the cells sharing one pass of it -- one
     # cell for the work lists (16-byte node records, 8-byte entries), 64
-    # cells for the tiles (node records and FP64 (r, w) 32 B per node, 16-byte
-    # entries)
-    alg_wl = alg_t = 0.0
+    # cells for the tiles (node records and FP64 (r, w) 32 B per node, plus
+    # the FP32 (r, w) of the mixed form; 16-byte entries of the tile CSR,
+    # every entry but each node's alpha one)
+    alg_wl = alg_t = alg_m = 0.0
     for i, c in enumerate(ctxs):
         k = int(np.sum(pick == i))
         inf = c.korobov_info()
+        tent = inf["entries"] - inf["nodes"]
         alg_wl += k * (16.0 * c.n_pnt + 16.0 * inf["nodes"] + 8.0 * inf["entries"])
-        alg_t += k * (16.0 * c.n_pnt + (32.0 * inf["nodes"] + 16.0 * inf["entries"]) / 64.0)
+        alg_t += k * (16.0 * c.n_pnt + (32.0 * inf["nodes"] + 16.0 * tent) / 64.0)
+        alg_m += k * (16.0 * c.n_pnt + (40.0 * inf["nodes"] + 16.0 * tent) / 64.0)
 
     def roof(alg, ms):
         gbs = alg / (ms * 1e-3) / 1e9
@@ -146,7 +149,7 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
                 "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
     tiles["fp64_tiles"]["roofline"] = roof(alg_t, tiles["fp64_tiles"]["ms_per_step"])
-    tiles["mixed_fp32_tiles"]["roofline"] = roof(alg_t, tiles["mixed_fp32_tiles"]["ms_per_step"])
+    tiles["mixed_fp32_tiles"]["roofline"] = roof(alg_m, tiles["mixed_fp32_tiles"]["ms_per_step"])
     out = {"cells": cells, "grids": {"%d^3" % n: int(np.sum(pick == i)) for i, n in enumerate(sizes)},
            "korobov_M": 1 << 20, "steps": steps, "evals_per_s": round(cells * steps / el, 1),
            "roofline": roof(alg_wl, 1e3 * el / steps), "cell_tiles": tiles,
@@ -458,11 +461,14 @@ def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed
     # algorithmic bytes of one evaluation: f in and f out (16 N) plus the
     # stencil over the cells that share one pass of it -- one cell for the
     # grouped kernels (16-byte node records, 8-byte entries), 64 for the
-    # tiles (node records and FP64 (r, w), 32 B per node; 16-byte entries)
+    # tiles (node records and FP64 (r, w), 32 B per node, plus the FP32
+    # (r, w) of the mixed form; 16-byte entries of the tile CSR, which holds
+    # every entry but the alpha one of each node: the node kernel sums those)
     inf = ctx.korobov_info()
+    tent = inf["entries"] - inf["nodes"]
     alg = {"plain_fp64_grouped": 16.0 * n + 16.0 * inf["nodes"] + 8.0 * inf["entries"],
-           "plain_fp64_tiles": 16.0 * n + (32.0 * inf["nodes"] + 16.0 * inf["entries"]) / 64.0,
-           "mixed_fp32_tiles": 16.0 * n + (32.0 * inf["nodes"] + 16.0 * inf["entries"]) / 64.0}
+           "plain_fp64_tiles": 16.0 * n + (32.0 * inf["nodes"] + 16.0 * tent) / 64.0,
+           "mixed_fp32_tiles": 16.0 * n + (40.0 * inf["nodes"] + 16.0 * tent) / 64.0}
     for name, b_ in alg.items():
         gbs = b_ * out[name]["evals_per_s"] / 1e9
         out[name]["roofline"] = {"bound": "hbm", "alg_bytes_per_eval": int(b_), "achieved": round(gbs, 1),

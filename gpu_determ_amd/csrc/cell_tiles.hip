@@ -60,7 +60,12 @@
 
 #define CT_THREADS 256
 #define CT_MOM_PARTS 64   // destination ranges (moment partials) per tile
-#define CT_UNROLL 8       // entries whose loads a lane issues together
+// entries whose loads a lane of the gather issues together: 12 FP64 rows
+// (2.216 -> 2.137 us per cell against 8; 16: 2.153), 8 FP32 rows (12: 1.333
+// -> 1.357, 16: 1.50)
+template <class T> struct CtUnroll {
+    static constexpr int value = sizeof(T) == 8 ? 12 : 8;
+};
 
 template <class T> struct CtK;
 template <> struct CtK<double> {
@@ -310,19 +315,27 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const 
 // role 0, beta) and l += 2 w f_alpha over the role-0 entries; L = f_d l.
 // FP64 products and sums; the FP64 tiles keep the single-cell path's term
 // for q (coef D, then the sum), the mixed form one FMA per term.  Each round issues
-// the loads of CT_UNROLL entries (records, then D values and f values) before
+// the loads of CtUnroll<T> entries (records, then D values and f values) before
 // adding any of them, with no branch between the loads: every entry loads a D
 // value and an f value (its partner's for role 0, the destination's own --
 // one cached row -- otherwise); entries past the destination's end are
 // clamped to its last one and add exact zeros (+-0 leaves a sum unchanged).
-// A node's D row is read by destinations all over the grid, so no order of
-// the destinations finds it in L2 again: grid order gives the same time as
-// longest first (1.62 us per cell, mixed), contiguous slabs of the grid per
-// XCD 1.68, longest first within such slabs 2.16 (uneven waves).
+// Destination order: with a grid of a multiple of 8 blocks (workgroups are
+// dealt to the XCDs round robin), block b takes its slots from XCD b % 8's
+// list, one octant of the box in Morton order (xperm / xo), else from the
+// longest-first order dperm.  A node's row is read by lambda and mu and by
+// their grid neighbours lambda + s and mu - s, so on one XCD, working
+// through a compact part of the grid, the second read often finds the row
+// in L2: FP64 gather 2.475 -> 2.224 us per cell against longest first; the
+// FP32 gather is unchanged (1.33), its time is not set by the missed bytes.
+struct CtXcd {
+    int off[9];
+};
 template <class T, int K>
 __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, const int32_t *__restrict__ cptr,
                                                           const uint4 *__restrict__ cent,
                                                           const int32_t *__restrict__ dperm,
+                                                          const int32_t *__restrict__ xperm, CtXcd xo,
                                                           const T *__restrict__ Dv, const T *__restrict__ ft, int t0,
                                                           double *__restrict__ Q0, double *__restrict__ L)
 {
@@ -332,21 +345,27 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, cons
     const T *Dt = Dv + (size_t)blockIdx.y * nn * K;
     const T *fr = ft + (size_t)(t0 + blockIdx.y) * npts * K;
     const size_t to = (size_t)(t0 + blockIdx.y) * npts * K;
-    for (int s = blockIdx.x * NPB + g; s < npts; s += gridDim.x * NPB) {
-        const int d = dperm[s];
+    // grids of a multiple of 8 blocks: block b walks XCD b % 8's octant list
+    const bool xcd = (gridDim.x & 7u) == 0u;
+    const int x = blockIdx.x & 7, lo = xcd ? xo.off[x] : 0, cnt = xcd ? xo.off[x + 1] - lo : npts;
+    const int sb = xcd ? (int)(blockIdx.x >> 3) : (int)blockIdx.x, nb = xcd ? (int)(gridDim.x >> 3) : (int)gridDim.x;
+    const int32_t *perm = xcd ? xperm + lo : dperm;
+    for (int s = sb * NPB + g; s < cnt; s += nb * NPB) {
+        const int d = perm[s];
         const int e0 = cptr[d], e1 = cptr[d + 1];
         // the node kernel's sums of d's alpha run come first
         double q = Q0[to + (size_t)d * K + k], l = L[to + (size_t)d * K + k];
-        for (int e = e0; e < e1; e += CT_UNROLL) {
-            uint4 x[CT_UNROLL];
+        constexpr int U = CtUnroll<T>::value;
+        for (int e = e0; e < e1; e += U) {
+            uint4 x[U];
 #pragma unroll
-            for (int u = 0; u < CT_UNROLL; u++) {  // clamped to the destination's last entry: in bounds
+            for (int u = 0; u < U; u++) {  // clamped to the destination's last entry: in bounds
                 const uint32_t eu = (uint32_t)min(e + u, e1 - 1);
                 x[u] = K == WAVE ? cent[eu] : ld_off(cent, eu);
             }
-            T dv[CT_UNROLL], fv[CT_UNROLL];
+            T dv[U], fv[U];
 #pragma unroll
-            for (int u = 0; u < CT_UNROLL; u++) {
+            for (int u = 0; u < U; u++) {
                 dv[u] = ct_row<K>(Dt, x[u].x >> 2, k);
                 // a wave per destination: the role is uniform, so only role-0
                 // entries load their partner's f (no divergence)
@@ -360,7 +379,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, cons
                     fv[u] = ct_row<K>(fr, x[u].y, k);
             }
 #pragma unroll
-            for (int u = 0; u < CT_UNROLL; u++) {
+            for (int u = 0; u < U; u++) {
                 const bool in = e + u < e1, role0 = (x[u].x & 3u) == 0u;
                 const double val = __hiloint2double((int)x[u].w, (int)x[u].z);
                 const double coef = in ? (role0 ? -1.0 : val) : 0.0;
@@ -526,6 +545,8 @@ static int ct_launch(gdt_context *h, const double *f, int cells, bool correct, d
     GDT_LAUNCH_TIMED(h, "ct_pack", (k_ct_pack<T, K>), dim3((np + 63) / 64, tiles), dim3(CT_THREADS), 0u, np, cells,
                      f, ft);
     const int nblk = std::max(1, h->ct_node_blocks), gblk = std::max(1, h->ct_gather_blocks);
+    CtXcd xo;
+    for (int i = 0; i < 9; i++) xo.off[i] = h->kxoff[i];
     for (int t0 = 0; t0 < tiles; t0 += chunk) {
         const int nt = std::min(chunk, tiles - t0);
         const auto *nrw = (const typename CtRW<T>::type *)(sizeof(T) == 8 ? h->krw.p : h->krwf.p);
@@ -533,8 +554,8 @@ static int ct_launch(gdt_context *h, const double *f, int cells, bool correct, d
                          h->kaptr.as<int32_t>(), h->kaperm.as<int32_t>(), h->kt4.as<uint4>(), nrw,
                          h->krw.as<double2>(), (const T *)ft, t0, Dv, Q0, L, h->kmath.as<double>());
         GDT_LAUNCH_TIMED(h, "ct_gather", (k_ct_gather<T, K>), dim3(gblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
-                         h->kcptr.as<int32_t>(), h->kcent.as<uint4>(), h->kdperm.as<int32_t>(), (const T *)Dv,
-                         (const T *)ft, t0, Q0, L);
+                         h->kcptr.as<int32_t>(), h->kcent.as<uint4>(), h->kdperm.as<int32_t>(),
+                         h->kxperm.as<int32_t>(), xo, (const T *)Dv, (const T *)ft, t0, Q0, L);
     }
     double *X = nullptr;
     if (correct) {
@@ -568,7 +589,7 @@ int ct_run(gdt_context *h, bool mixed, const double *f, int cells, bool correct,
            double *nu_out, double dt, double *f_upd)
 {
     GDT_CHECK(h->kcent.p && h->kcptr.p && h->kdperm.p && h->krw.p && h->krwf.p && h->kt4.p && h->kaptr.p &&
-                  h->kaperm.p, GDT_E_STATE,
+                  h->kaperm.p && h->kxperm.p, GDT_E_STATE,
               "cell tiles: stencil not built");
     if (h->ct_wide)
         return mixed ? ct_launch<float, WAVE>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd)

@@ -773,13 +773,34 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                              [&](int32_t x, int32_t y) { return cp[x + 1] - cp[x] > cp[y + 1] - cp[y]; });
             std::stable_sort(aperm.begin(), aperm.end(),
                              [&](int32_t x, int32_t y) { return aptr[x + 1] - aptr[x] > aptr[y + 1] - aptr[y]; });
+            // per XCD, one octant of the box (index halves), in Morton order:
+            // the destinations a XCD gathers at one time are a compact part of
+            // the grid, so a node's row read for lambda is still in that
+            // XCD's L2 when the neighbour lambda + s reads it (and mu, mu - s)
+            std::vector<int32_t> xperm(np);
+            std::vector<uint64_t> xkey(np);
+            for (int v = 0; v < np; v++) {
+                const int i = h->h_cidx[3 * v], j = h->h_cidx[3 * v + 1], k = h->h_cidx[3 * v + 2];
+                const uint64_t oct = (uint64_t)((i >= h->n[0] / 2) * 4 + (j >= h->n[1] / 2) * 2 + (k >= h->n[2] / 2));
+                uint64_t mc = 0;
+                for (int bit = 0; bit < 15; bit++)
+                    mc |= (((uint64_t)(i >> bit) & 1u) << (3 * bit + 2)) | (((uint64_t)(j >> bit) & 1u) << (3 * bit + 1)) |
+                          (((uint64_t)(k >> bit) & 1u) << (3 * bit));
+                xkey[v] = (oct << 48) | mc;
+                xperm[v] = v;
+            }
+            std::stable_sort(xperm.begin(), xperm.end(), [&](int32_t x, int32_t y) { return xkey[x] < xkey[y]; });
+            for (int x = 0; x <= 8; x++) h->kxoff[x] = 0;
+            for (int v = 0; v < np; v++) h->kxoff[(xkey[v] >> 48) + 1]++;
+            for (int x = 0; x < 8; x++) h->kxoff[x + 1] += h->kxoff[x];
             GDT_CHECK(h->krw.ensure(sizeof(double) * 2 * nn) == 0 && h->krwf.ensure(sizeof(float) * 2 * nn) == 0 &&
                           h->kt4.ensure(sizeof(uint32_t) * 4 * nn) == 0 &&
                           h->kaptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
                           h->kaperm.ensure(sizeof(int32_t) * np) == 0 &&
                           h->kcptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
                           h->kcent.ensure(sizeof(uint32_t) * 4 * std::max(nte, 1L)) == 0 &&
-                          h->kdperm.ensure(sizeof(int32_t) * np) == 0,
+                          h->kdperm.ensure(sizeof(int32_t) * np) == 0 &&
+                          h->kxperm.ensure(sizeof(int32_t) * np) == 0,
                       GDT_E_NOMEM, "gdt_korobov_build: cell-tile stencil allocation failed");
             GDT_HIP(hipMemcpy(h->krw.p, rw.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice));
             GDT_HIP(hipMemcpy(h->krwf.p, rwf.data(), sizeof(float) * 2 * nn, hipMemcpyHostToDevice));
@@ -790,6 +811,7 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
             if (nte > 0)
                 GDT_HIP(hipMemcpy(h->kcent.p, cent.data(), sizeof(uint32_t) * 4 * nte, hipMemcpyHostToDevice));
             GDT_HIP(hipMemcpy(h->kdperm.p, dperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+            GDT_HIP(hipMemcpy(h->kxperm.p, xperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
             h->ntent = nte;
         }
         h->h_gp.swap(gp);

@@ -39,8 +39,7 @@ void gdt_set_error(int code, const char *fmt, ...);
 #define KOR_NPTS_MAX 19200 // grid nodes the packed node records and the LDS copy of f admit
 #define KOR_APPLY_THREADS 128 // destinations per apply block (128: 26.0 us per step; 64: 27.0, 256: 26.3, 512: 27.9)
 
-// Device buffer owned by a context.
-// Owns one device allocation: freed by release() or on destruction (so an
+// Device buffer: owns one device allocation: freed by release() or on destruction (so an
 // early return or a destroyed context frees every buffer); movable, not
 // copyable.
 struct DevBuf {
@@ -125,6 +124,11 @@ struct gdt_context {
     // the destinations ordered by its entry count, longest first
     DevBuf kt4, kaptr, kaperm;
     DevBuf kcptr, kcent, kdperm;
+    // the gather's destinations per XCD: the grid's octant x (of the box's
+    // index halves) in Morton order, for the blocks on XCD x; xoff[x] ..
+    // xoff[x + 1] index kxperm
+    DevBuf kxperm;
+    int kxoff[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     DevBuf krw, krwf;             // (r, w) per tile node position, FP64 and FP32
     long ntent = 0;               // entries of the tile CSR
     int wb_blocks = 0;            // blocks per cell the wave bounds kwb are built for
