@@ -9,7 +9,8 @@
 * cached state that must follow its inputs: the step graph after the scratch
   it captured is reallocated, the matrix-core operands after the tables
   change, the surrogate's residual gate on cells whose sums are not finite,
-  and the matrix gain with more than 64 KiB of LDS.
+  and the matrix gain with more than 64 KiB of LDS;
+* destroyed contexts give back all their device memory.
 """
 import sys
 
@@ -188,3 +189,43 @@ def test_matrix_gain_22cube_large_lds():
     og = O.gain(g, oa, fb, dest)[0]
     assert rel_l2(nu.get(), onu) < 1e-13
     assert rel_l2(gain.get()[0, ::37], og) < 1e-13
+
+
+def test_destroyed_contexts_free_their_device_memory():
+    """A destroyed context frees every device buffer it grew, including the
+    cell-tile stencil and scratch (node-term rows of a chunk, f tiles, Q0 / L
+    rows): after a first round (which may leave the runtime's own one-time
+    allocations behind: code objects, kernel-argument pools), creating, using
+    and destroying a context again leaves the device's free memory where it
+    was (within 16 MiB per round), while one context holds over a GiB."""
+    import ctypes
+    import gc
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+
+    def free_bytes():
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+        return free.value
+
+    g = O.Grid((24,) * 3, (-VM,) * 3, (VM,) * 3, VM)
+    rng = np.random.default_rng(23)
+    f = np.stack([mixture(g.xi, rng) for _ in range(130)])
+    start, held, after = free_bytes(), [], []
+    for it in range(4):
+        c = G.Context(g.n, g.vmin, g.vmax, g.R)
+        c.korobov_build(1 << 20, seed=1, s0=0.25)
+        fd = G.DeviceArray.from_host(f)
+        for fl in (G.FLAG_CELL_TILES, G.FLAG_MIXED, 0):
+            out = c.evaluate(G.METHOD_KOROBOV, fd, f.shape[0], G.FLAG_CORRECT | fl)
+            del out
+        c.sync()
+        held.append(start - free_bytes())
+        del fd
+        c.close()
+        del c
+        gc.collect()
+        after.append(start - free_bytes())
+        say("context round %d: held %.1f MiB, left %.1f MiB" % (it, held[-1] / 2**20, after[-1] / 2**20))
+    assert min(held) > 1 << 30
+    assert max(after[1:]) - after[0] < 16 << 20
