@@ -63,6 +63,9 @@
 // entries whose loads a lane of the gather issues together: 12 FP64 rows
 // (2.216 -> 2.137 us per cell against 8; 16: 2.153), 8 FP32 rows (12: 1.333
 // -> 1.357, 16: 1.50)
+// entries whose D rows a lane of the FP32 gather loads together past the beta
+// run (16: 1.088 us per cell; 12: 1.100; 8: 1.104)
+#define CT_UNROLL_D 16
 template <class T> struct CtUnroll {
     static constexpr int value = sizeof(T) == 8 ? 12 : 8;
 };
@@ -333,6 +336,7 @@ struct CtXcd {
 };
 template <class T, int K>
 __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, const int32_t *__restrict__ cptr,
+                                                          const int32_t *__restrict__ cmid,
                                                           const uint4 *__restrict__ cent,
                                                           const int32_t *__restrict__ dperm,
                                                           const int32_t *__restrict__ xperm, CtXcd xo,
@@ -352,39 +356,80 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, cons
     const int32_t *perm = xcd ? xperm + lo : dperm;
     for (int s = sb * NPB + g; s < cnt; s += nb * NPB) {
         const int d = perm[s];
-        const int e0 = cptr[d], e1 = cptr[d + 1];
+        const int e0 = cptr[d], em = cmid[d], e1 = cptr[d + 1];
         // the node kernel's sums of d's alpha run come first
         double q = Q0[to + (size_t)d * K + k], l = L[to + (size_t)d * K + k];
         constexpr int U = CtUnroll<T>::value;
-        for (int e = e0; e < e1; e += U) {
-            uint4 x[U];
+        if (sizeof(T) == 8) {
+            // FP64 rows: one pass over [e0, e1); with a wave per destination
+            // the role is uniform, so only the beta entries load their
+            // partner's f row (no divergence; narrow tiles load every row).
+            // The two-run form below is slower here (2.22 against 2.15 us
+            // per cell).
+            for (int e = e0; e < e1; e += U) {
+                uint4 x[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {  // clamped to the destination's last entry: in bounds
-                const uint32_t eu = (uint32_t)min(e + u, e1 - 1);
-                x[u] = K == WAVE ? cent[eu] : ld_off(cent, eu);
+                for (int u = 0; u < U; u++) {  // clamped to the destination's last entry: in bounds
+                    const uint32_t eu = (uint32_t)min(e + u, e1 - 1);
+                    x[u] = K == WAVE ? cent[eu] : ld_off(cent, eu);
+                }
+                T dv[U], fv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    dv[u] = ct_row<K>(Dt, x[u].x >> 2, k);
+                    fv[u] = (K != WAVE || (x[u].x & 3u) == 0u) ? ct_row<K>(fr, x[u].y, k) : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool in = e + u < e1, role0 = (x[u].x & 3u) == 0u;
+                    const double val = __hiloint2double((int)x[u].w, (int)x[u].z);
+                    q = ct_acc(q, in ? (role0 ? -1.0 : val) : 0.0, dv[u]);
+                    l += (in && role0) ? val * (double)fv[u] : 0.0;
+                }
             }
-            T dv[U], fv[U];
+        } else {
+            // FP32 rows: the beta entries [e0, em) load the node's D row and
+            // the partner's f row, the others D rows only -- no branch and no
+            // f row for the other 79% of the entries (1.34 -> 1.10 us per
+            // cell against one pass loading every entry's f row)
+            for (int e = e0; e < em; e += U) {
+                uint4 x[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                dv[u] = ct_row<K>(Dt, x[u].x >> 2, k);
-                // a wave per destination: the role is uniform, so only role-0
-                // entries load their partner's f (no divergence)
-                // FP64 rows with a wave per destination: the role is uniform,
-                // so only role-0 entries load their partner's f (3% faster);
-                // FP32: every entry loads (the branches cost more than the
-                // cached loads: 1.85 against 1.62 us per cell)
-                if (K == WAVE && sizeof(T) == 8)
-                    fv[u] = (x[u].x & 3u) == 0u ? ct_row<K>(fr, x[u].y, k) : (T)0;
-                else
+                for (int u = 0; u < U; u++) {  // clamped to the run's last entry: in bounds
+                    const uint32_t eu = (uint32_t)min(e + u, em - 1);
+                    x[u] = K == WAVE ? cent[eu] : ld_off(cent, eu);
+                }
+                T dv[U], fv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    dv[u] = ct_row<K>(Dt, x[u].x >> 2, k);
                     fv[u] = ct_row<K>(fr, x[u].y, k);
-            }
+                }
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const bool in = e + u < e1, role0 = (x[u].x & 3u) == 0u;
-                const double val = __hiloint2double((int)x[u].w, (int)x[u].z);
-                const double coef = in ? (role0 ? -1.0 : val) : 0.0;
-                q = ct_acc(q, coef, dv[u]);
-                l += (in && role0) ? val * (double)fv[u] : 0.0;
+                for (int u = 0; u < U; u++) {
+                    const bool in = e + u < em;
+                    const double val = __hiloint2double((int)x[u].w, (int)x[u].z);
+                    q = ct_acc(q, in ? -1.0 : 0.0, dv[u]);
+                    l += in ? val * (double)fv[u] : 0.0;
+                }
+            }
+            // lambda, mu, lambda+s, mu-s entries [em, e1): D rows only
+            constexpr int U2 = CT_UNROLL_D;
+            for (int e = em; e < e1; e += U2) {
+                uint4 x[U2];
+#pragma unroll
+                for (int u = 0; u < U2; u++) {  // clamped to the destination's last entry: in bounds
+                    const uint32_t eu = (uint32_t)min(e + u, e1 - 1);
+                    x[u] = K == WAVE ? cent[eu] : ld_off(cent, eu);
+                }
+                T dv[U2];
+#pragma unroll
+                for (int u = 0; u < U2; u++) dv[u] = ct_row<K>(Dt, x[u].x >> 2, k);
+#pragma unroll
+                for (int u = 0; u < U2; u++) {
+                    const double val = __hiloint2double((int)x[u].w, (int)x[u].z);
+                    q = ct_acc(q, e + u < e1 ? val : 0.0, dv[u]);
+                }
             }
         }
         Q0[to + (size_t)d * K + k] = q;
@@ -554,7 +599,8 @@ static int ct_launch(gdt_context *h, const double *f, int cells, bool correct, d
                          h->kaptr.as<int32_t>(), h->kaperm.as<int32_t>(), h->kt4.as<uint4>(), nrw,
                          h->krw.as<double2>(), (const T *)ft, t0, Dv, Q0, L, h->kmath.as<double>());
         GDT_LAUNCH_TIMED(h, "ct_gather", (k_ct_gather<T, K>), dim3(gblk, nt), dim3(CT_THREADS), 0u, np, (int)h->nn,
-                         h->kcptr.as<int32_t>(), h->kcent.as<uint4>(), h->kdperm.as<int32_t>(),
+                         h->kcptr.as<int32_t>(), h->kcmid.as<int32_t>(),
+                         (sizeof(T) == 8 ? h->kcent : h->kcent2).as<uint4>(), h->kdperm.as<int32_t>(),
                          h->kxperm.as<int32_t>(), xo, (const T *)Dv, (const T *)ft, t0, Q0, L);
     }
     double *X = nullptr;
@@ -589,7 +635,7 @@ int ct_run(gdt_context *h, bool mixed, const double *f, int cells, bool correct,
            double *nu_out, double dt, double *f_upd)
 {
     GDT_CHECK(h->kcent.p && h->kcptr.p && h->kdperm.p && h->krw.p && h->krwf.p && h->kt4.p && h->kaptr.p &&
-                  h->kaperm.p && h->kxperm.p, GDT_E_STATE,
+                  h->kaperm.p && h->kxperm.p && h->kcmid.p && h->kcent2.p, GDT_E_STATE,
               "cell tiles: stencil not built");
     if (h->ct_wide)
         return mixed ? ct_launch<float, WAVE>(h, f, cells, correct, Qout, Lout, nu_out, dt, f_upd)

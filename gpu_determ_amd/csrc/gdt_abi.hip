@@ -730,26 +730,42 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
             //   role 1 / 2:     g = the destination itself, value = 1 - r / r,
             //                   r = (num / 4) / (den / 4) -- the quotient the
             //                   group gather forms
-            std::vector<uint32_t> cent(4 * (size_t)nte);
-            std::vector<int32_t> cf(cp.begin(), cp.end() - 1);
-            for (long q = 0; q < nn; q++) {
+            // Two orders of the same entries within each destination's segment:
+            //   cent   (q, slot) order, read by the FP64 gather;
+            //   cent2  the beta entries first (in q order), then the others in
+            //          (q, slot) order; cmid[v] is where the others begin, so
+            //          the FP32 gather loads a partner's f row in the first
+            //          run only.
+            std::vector<uint32_t> cent(4 * (size_t)nte), cent2(4 * (size_t)nte);
+            std::vector<int32_t> cmid(np);
+            auto put = [&](std::vector<uint32_t> &dst_ent, std::vector<int32_t> &fillp, long q, int sl) {
                 const long t = tord[q];
                 const uint32_t y = t4[4 * (size_t)q + 3];
                 const double num4 = (double)(y & 0x7FFu), den4 = (double)((y >> 11) & 0x7FFu);
                 const double r = num4 == 0.0 ? 0.0 : num4 / den4;
-                for (int sl = 1; sl < slots(t); sl++) {
-                    const uint32_t role = sl < 2 ? 0u : (sl < 4 ? 1u : 2u);
-                    const int32_t dst = cidx6[6 * t + sl];
-                    const double val = role == 0u ? 2.0 * cw[t] : (role == 1u ? 1.0 - r : r);
-                    const int32_t g = role == 0u ? cidx6[6 * t] : dst;
-                    uint64_t cb;
-                    std::memcpy(&cb, &val, sizeof cb);
-                    const int32_t pos = cf[dst]++;
-                    cent[4 * (size_t)pos] = ((uint32_t)q << 2) | role;
-                    cent[4 * (size_t)pos + 1] = (uint32_t)g;
-                    cent[4 * (size_t)pos + 2] = (uint32_t)cb;
-                    cent[4 * (size_t)pos + 3] = (uint32_t)(cb >> 32);
-                }
+                const uint32_t role = sl < 2 ? 0u : (sl < 4 ? 1u : 2u);
+                const int32_t dst = cidx6[6 * t + sl];
+                const double val = role == 0u ? 2.0 * cw[t] : (role == 1u ? 1.0 - r : r);
+                const int32_t g = role == 0u ? cidx6[6 * t] : dst;
+                uint64_t cb;
+                std::memcpy(&cb, &val, sizeof cb);
+                const int32_t pos = fillp[dst]++;
+                dst_ent[4 * (size_t)pos] = ((uint32_t)q << 2) | role;
+                dst_ent[4 * (size_t)pos + 1] = (uint32_t)g;
+                dst_ent[4 * (size_t)pos + 2] = (uint32_t)cb;
+                dst_ent[4 * (size_t)pos + 3] = (uint32_t)(cb >> 32);
+            };
+            {
+                std::vector<int32_t> cf(cp.begin(), cp.end() - 1);
+                for (long q = 0; q < nn; q++)
+                    for (int sl = 1; sl < slots(tord[q]); sl++) put(cent, cf, q, sl);
+            }
+            {
+                std::vector<int32_t> cf(cp.begin(), cp.end() - 1);
+                for (long q = 0; q < nn; q++) put(cent2, cf, q, 1);
+                for (int v = 0; v < np; v++) cmid[v] = cf[v];
+                for (long q = 0; q < nn; q++)
+                    for (int sl = 2; sl < slots(tord[q]); sl++) put(cent2, cf, q, sl);
             }
             // per tile position: r and w of the node (FP64, and rounded to
             // FP32 for the mixed node terms) -- r the quotient num / den the
@@ -798,7 +814,9 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                           h->kaptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
                           h->kaperm.ensure(sizeof(int32_t) * np) == 0 &&
                           h->kcptr.ensure(sizeof(int32_t) * (np + 1)) == 0 &&
+                          h->kcmid.ensure(sizeof(int32_t) * np) == 0 &&
                           h->kcent.ensure(sizeof(uint32_t) * 4 * std::max(nte, 1L)) == 0 &&
+                          h->kcent2.ensure(sizeof(uint32_t) * 4 * std::max(nte, 1L)) == 0 &&
                           h->kdperm.ensure(sizeof(int32_t) * np) == 0 &&
                           h->kxperm.ensure(sizeof(int32_t) * np) == 0,
                       GDT_E_NOMEM, "gdt_korobov_build: cell-tile stencil allocation failed");
@@ -808,8 +826,11 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
             GDT_HIP(hipMemcpy(h->kaptr.p, aptr.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice));
             GDT_HIP(hipMemcpy(h->kaperm.p, aperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
             GDT_HIP(hipMemcpy(h->kcptr.p, cp.data(), sizeof(int32_t) * (np + 1), hipMemcpyHostToDevice));
-            if (nte > 0)
+            GDT_HIP(hipMemcpy(h->kcmid.p, cmid.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+            if (nte > 0) {
                 GDT_HIP(hipMemcpy(h->kcent.p, cent.data(), sizeof(uint32_t) * 4 * nte, hipMemcpyHostToDevice));
+                GDT_HIP(hipMemcpy(h->kcent2.p, cent2.data(), sizeof(uint32_t) * 4 * nte, hipMemcpyHostToDevice));
+            }
             GDT_HIP(hipMemcpy(h->kdperm.p, dperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
             GDT_HIP(hipMemcpy(h->kxperm.p, xperm.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
             h->ntent = nte;

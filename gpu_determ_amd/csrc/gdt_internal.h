@@ -124,6 +124,7 @@ struct gdt_context {
     // the destinations ordered by its entry count, longest first
     DevBuf kt4, kaptr, kaperm;
     DevBuf kcptr, kcent, kdperm;
+    DevBuf kcent2, kcmid;         // the tile CSR's entries, beta first per destination; where the others begin
     // the gather's destinations per XCD: the grid's octant x (of the box's
     // index halves) in Morton order, for the blocks on XCD x; xoff[x] ..
     // xoff[x + 1] index kxperm
