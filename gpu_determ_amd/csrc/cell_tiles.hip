@@ -567,6 +567,7 @@ int ct_prepare(gdt_context *h, int cells, bool mixed)
     const long tiles = (cells + K - 1) / K;
     const long chunk = std::min<long>(tiles, h->ct_chunk_tiles);
     const size_t np = h->n_pnt;
+    GDT_CHECK_GRID_Y(tiles, "cell tiles: tiles of cells");
     // node values of a chunk and an f tile are indexed with 32-bit byte offsets
     GDT_CHECK((double)sz * chunk * h->nn * K < 4294967296.0 && h->nn < (1L << 30), GDT_E_ARG,
               "cell tiles: %ld node values per chunk exceed the 32-bit offsets", chunk * h->nn * K);

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void k_loss_generic(int nu, int nv, int nw, co
 
 int launch_loss(gdt_context *h, const double *f_box, double *nu_box, int cells)
 {
+    GDT_CHECK_GRID_Y(cells, "loss: cells");
     const int nu = h->n[0], nv = h->n[1], nw = h->n[2];
     const dim3 grid((nu * nv + 63) / 64, cells);
     const double *b = h->tb.as<double>();
@@ -244,6 +245,7 @@ int gain_prepare(gdt_context *h)
 
 int launch_gain(gdt_context *h, const double *f_box, double *gain_box, int cells)
 {
+    GDT_CHECK_GRID_Y(cells, "gain: cells");
     const int nu = h->n[0], nv = h->n[1], nw = h->n[2];
     const int blocks = ((nu + 3) / 4) * ((nv + 3) / 4) * ((nw + 3) / 4);
     const size_t lds = sizeof(double) * nu * nv * gain_row(nw);

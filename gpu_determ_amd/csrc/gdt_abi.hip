@@ -936,6 +936,7 @@ static bool use_tiles(unsigned flags) { return (flags & (GDT_FLAG_CELL_TILES | G
 // group partials, and the fused path's wave partition (outside any capture).
 static int kor_prepare(gdt_context *h, int cells)
 {
+    GDT_CHECK_GRID_Y(cells, "Korobov: cells");
     const int nb = std::max(kor_blocks(h, cells), kor_gather_parts(h));
     const size_t mp = sizeof(double) * 20 * (size_t)cells * nb;
     const size_t dw = sizeof(double) * 2 * (size_t)h->nn * cells;

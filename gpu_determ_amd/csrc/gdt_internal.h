@@ -37,6 +37,14 @@ void gdt_set_error(int code, const char *fmt, ...);
 #define KOR_TILES_MAX 64  // default gather tiles per group: G * 64 blocks, two per CU, one round
 #define KOR_TILES_LIMIT 1024
 #define KOR_NPTS_MAX 19200 // grid nodes the packed node records and the LDS copy of f admit
+// Cells (grouped Korobov and analytic-matrix paths) or 64-cell tiles (cell
+// tiles) one launch takes: they index grid y, which admits at most 65,535.
+#define GDT_GRID_Y_MAX 65535
+#define GDT_CHECK_GRID_Y(n, what)                                                                            \
+    GDT_CHECK((n) >= 1 && (n) <= GDT_GRID_Y_MAX, GDT_E_ARG,                                                  \
+              "%s: %ld per call is outside the launch limit of 1 to 65,535 (grid y); split the cells into " \
+              "several calls",                                                                                \
+              (what), (long)(n))
 #define KOR_APPLY_THREADS 128 // destinations per apply block (128: 26.0 us per step; 64: 27.0, 256: 26.3, 512: 27.9)
 
 // Device buffer: owns one device allocation: freed by release() or on destruction (so an

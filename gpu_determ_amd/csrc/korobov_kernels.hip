@@ -802,6 +802,7 @@ int kor_setup(gdt_context *h, int cells)
 
 int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
 {
+    GDT_CHECK_GRID_Y(cells, "kor_node: cells");
     const int B = kor_blocks(h, cells);
     GDT_LAUNCH_TIMED(h, "kor_node", k_kor_node, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->n_pnt, h->nn,
                      h->kn4.as<uint4>(), h->kK, f, (double2 *)DW, h->kmath.as<double>());
@@ -820,6 +821,7 @@ int kor_gather_blocks(gdt_context *h, int cells)
 
 int launch_kor_gather(gdt_context *h, const double *V, double *Lpart, double *Qpart, double *mpart, int cells)
 {
+    GDT_CHECK_GRID_Y(cells, "kor_gather: cells");
     const int B = kor_gather_blocks(h, cells);
     GDT_LAUNCH_TIMED(h, "kor_gather", k_kor_gather, dim3(B, cells), dim3(TILE_THREADS),
                      (uint32_t)(sizeof(double2) * h->kcap), h->n_pnt, h->nn, h->kntile, h->kcap,
@@ -831,6 +833,7 @@ int launch_kor_gather(gdt_context *h, const double *V, double *Lpart, double *Qp
 
 int launch_kor_fused(gdt_context *h, const double *f, double *loss, double *Q, double *mpart, int cells)
 {
+    GDT_CHECK_GRID_Y(cells, "kor_fused: cells");
     const int B = kor_blocks(h, cells);
     GDT_CHECK(h->wb_blocks == B, GDT_E_STATE, "fused: wave bounds for %d blocks not prepared", B);
     GDT_LAUNCH_TIMED(h, "kor_fused", k_kor_fused, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->n_pnt,
@@ -844,6 +847,7 @@ int launch_kor_apply(gdt_context *h, bool correct, int G, int nparts, const doub
                      const double *Lpart, const double *Qpart, double *Qout, double *Lout, double *nu_out, double *X,
                      int cells, double dt, double *f_upd)
 {
+    GDT_CHECK_GRID_Y(cells, "kor_apply: cells");
     GDT_CHECK(G == 1 || G == KOR_GROUPS, GDT_E_ARG, "kor_apply: %d partials per node", G);
     const dim3 grid((h->n_pnt + APPLY_THREADS - 1) / APPLY_THREADS, cells);
     auto kern = correct ? (G == 1 ? k_kor_apply<true, 1> : k_kor_apply<true, KOR_GROUPS>)

@@ -168,6 +168,11 @@ int gdt_korobov_export_csr(gdt_handle h, int32_t *ptr, uint32_t *ent);
 uint32_t gdt_korobov_pick(uint32_t M, uint64_t seed, int ncand, double *p2);
 
 /* ---- collision operator ------------------------------------------------- */
+/* Cells per call: at most 65,535 on the analytic-matrix path and the grouped
+ * Korobov path (one launch holds every cell), and 65,535 tiles of 64 cells
+ * with GDT_FLAG_CELL_TILES / GDT_FLAG_MIXED; more return GDT_E_ARG (split
+ * the cells into several calls).  The reference runs 256 cells per launch
+ * (npx, collision_integral.cu:589). */
 /* Matrix method: nu (loss frequency, evolve_direct_colisions,
  * collision_integral.cu:32-72) and gain (evolve_inverse_colisions, :75-188,
  * all 512 boxes, intended output slot) for `cells` cells.  Either output may
