@@ -66,6 +66,10 @@
 // entries whose D rows a lane of the FP32 gather loads together past the beta
 // run (16: 1.088 us per cell; 12: 1.100; 8: 1.104)
 #define CT_UNROLL_D 16
+// nodes of an alpha run a wave evaluates together in the node kernel, their
+// loads in flight at once (us per cell FP64 / FP32: 4: 1.726 / 0.732;
+// 2: 1.725 / 0.797; 3: 1.750 / 0.762; 6: 1.726 / 0.776; 8: 1.92 / 0.92)
+#define CT_NODE_UNROLL 4
 template <class T> struct CtUnroll {
     static constexpr int value = sizeof(T) == 8 ? 12 : 8;
 };
@@ -228,8 +232,9 @@ __device__ __forceinline__ T ct_row(const T *__restrict__ fr, uint32_t i, int k)
 // destination slots s = x NPB + g, + gridDim.x NPB, ... (NPB = CT_THREADS / K)
 // of the longest-first order aperm; lane k of the group is cell k of the
 // tile.  For destination a the group walks a's alpha run, positions
-// [aptr[a], aptr[a + 1]) of the tile node order, two nodes per pass (their
-// loads in flight together): it stores each node's D row, and sums the run's
+// [aptr[a], aptr[a + 1]) of the tile node order, CT_NODE_UNROLL nodes per
+// pass (their loads in flight together): it stores each node's D row, and
+// sums the run's
 // alpha terms of Q0 (coefficient -1) and of the loss (2 w f_beta, FP64 w)
 // in the CSR's term form; the sums start the gather's sums of destination a
 // (Q0 and L of the tile, L before its factor f_a).  Record loads are one
@@ -286,21 +291,26 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const 
         const int q0 = aptr[a], q1 = aptr[a + 1];
         const T fa = ld((uint32_t)a);
         double qs = 0.0, ls = 0.0;
-        for (int q = q0; q < q1; q += 2) {
-            const bool two = q + 1 < q1;
-            const int qb = two ? q + 1 : q;  // clamped: in bounds
-            const uint4 x = nrec[q], x2 = nrec[qb];
-            const CtF<T> v = ct_load(x, fa, ld), v2 = ct_load(x2, fa, ld);
-            const double w1 = rw64[q].y, w2 = rw64[qb].y;
-            const T d = ct_terms(x, nrw[q], v, ts);
-            const T d2 = ct_terms(x2, nrw[qb], v2, ts);
-            store(q, d);
-            qs = ct_acc(qs, -1.0, d);
-            ls += (2.0 * w1) * (double)v.b;
-            if (two) {
-                store(qb, d2);
-                qs = ct_acc(qs, -1.0, d2);
-                ls += (2.0 * w2) * (double)v2.b;
+        constexpr int NU = CT_NODE_UNROLL;
+        for (int q = q0; q < q1; q += NU) {
+            uint4 x[NU];
+            int qq[NU];
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                qq[u] = q + u < q1 ? q + u : q1 - 1;  // clamped: in bounds
+                x[u] = nrec[qq[u]];
+            }
+            CtF<T> v[NU];
+#pragma unroll
+            for (int u = 0; u < NU; u++) v[u] = ct_load(x[u], fa, ld);
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                const T d = ct_terms(x[u], nrw[qq[u]], v[u], ts);
+                if (q + u < q1) {
+                    store(qq[u], d);
+                    qs = ct_acc(qs, -1.0, d);
+                    ls += (2.0 * rw64[qq[u]].y) * (double)v[u].b;
+                }
             }
         }
         Q0[to + (size_t)a * K + k] = qs;
