@@ -51,12 +51,15 @@ extern "C" {
                                   node's term per CSR entry, instead of the
                                   node-value + gather pair                  */
 #define GDT_FLAG_CELL_TILES 32u /* Korobov, many cells: cells interleaved in
-                                  tiles of 16 that share every stencil read
-                                  (FP64 node terms and sums)                */
-#define GDT_FLAG_MIXED 64u     /* Korobov, many cells: tiles of 32 cells with
-                                  f and the node terms in FP32, every sum and
-                                  the correction in FP64 (relative L2 error
-                                  of Q ~1e-6 .. 2e-5 against FP64)          */
+                                  tiles of 64 (16 in the narrow launch shape
+                                  of gdt_korobov_cell_tiles) that share every
+                                  stencil read (FP64 node terms and sums)   */
+#define GDT_FLAG_MIXED 64u     /* Korobov, many cells: the same tiles (32
+                                  cells in the narrow shape) with f and the
+                                  node terms in FP32, every sum and the
+                                  correction in FP64 (relative L2 error of Q
+                                  against FP64: about 1e-7 over many cells,
+                                  at most 1e-4 per cell)                    */
 
 typedef struct gdt_context *gdt_handle;
 

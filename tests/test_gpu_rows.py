@@ -11,7 +11,8 @@
   change, the surrogate's residual gate on cells whose sums are not finite,
   and the matrix gain with more than 64 KiB of LDS;
 * destroyed contexts give back all their device memory; calls beyond the
-  launch limit of 65,535 cells are refused with an error.
+  launch limit of 65,535 cells are refused with an error; bad arguments
+  return the ABI's error codes.
 """
 import sys
 
@@ -256,3 +257,41 @@ def test_cells_beyond_the_launch_limit_are_refused():
     fd2 = G.DeviceArray.from_host(np.tile(np.exp(-np.sum(g2.xi ** 2, axis=1) / 2), (65536, 1)))
     with pytest.raises(RuntimeError, match="65,535"):
         c2.collide_matrix(fd2, 65536)
+
+
+def test_argument_errors():
+    """The C ABI returns its error codes, with a message, instead of
+    launching on bad arguments: no cells, null arrays, an unknown method or
+    layout, a method whose tables or stencil are not built, negative steps,
+    an invalid lattice size or generator; the context stays usable."""
+    say("argument errors")
+    L = G.lib()
+    g = O.Grid((8,) * 3, (-VM,) * 3, (VM,) * 3, VM)
+    c = G.Context(g.n, g.vmin, g.vmax, g.R)
+    n = g.n_pnt
+    f0 = (maxwellian(g.xi, 0.6, 1.0, [0.8, 0, 0]) + maxwellian(g.xi, 0.4, 0.5, [-0.8, 0, 0]))[None]
+    f = G.DeviceArray.from_host(f0)
+    out = [G.DeviceArray((1, n)) for _ in range(3)]
+    ev = lambda method, fp, cells, layout=0, flags=G.FLAG_CORRECT: L.gdt_evaluate(
+        c.h, method, fp, out[0].ptr, out[1].ptr, out[2].ptr, cells, layout, flags)
+    E_ARG, E_STATE = -1, -3
+    assert ev(G.METHOD_KOROBOV, f.ptr, 1) == E_STATE  # stencil not built
+    assert ev(G.METHOD_MATRIX, f.ptr, 1) == E_STATE   # tables not built
+    assert "not built" in L.gdt_last_error().decode()
+    assert L.gdt_korobov_build(c.h, 3000, 1, 0, 0.25) == E_ARG      # M not a power of two
+    assert L.gdt_korobov_build(c.h, 1 << 12, 1, 6, 0.25) == E_ARG   # even generator
+    assert L.gdt_korobov_build(c.h, 1 << 12, 1, 0, -1.0) == E_ARG   # s0 <= 0
+    c.korobov_build(1 << 12, seed=1, s0=0.25)
+    assert ev(G.METHOD_KOROBOV, f.ptr, 0) == E_ARG
+    assert ev(G.METHOD_KOROBOV, None, 1) == E_ARG
+    assert ev(7, f.ptr, 1) == E_ARG
+    assert "method 7" in L.gdt_last_error().decode()
+    assert ev(G.METHOD_KOROBOV, f.ptr, 1, layout=5) == E_ARG
+    assert ev(G.METHOD_KOROBOV, f.ptr, -3, flags=G.FLAG_CORRECT | G.FLAG_CELL_TILES) == E_ARG
+    assert L.gdt_run_steps(c.h, G.METHOD_KOROBOV, f.ptr, 1, 0.02, -1, G.FLAG_CORRECT) == E_ARG
+    # still usable: the same evaluation as the oracle
+    assert ev(G.METHOD_KOROBOV, f.ptr, 1) == 0
+    st = O.korobov_stencil(g, 1 << 12, c.korobov_info()["a"], 0.25)
+    lo, q = O.korobov_collide(g, st, f0)
+    qc, _, _ = O.correct_loss(g.xi, lo, q, 1)
+    assert np.max(np.abs(out[2].get() - qc)) <= 1e-12 * np.max(np.abs(lo))
