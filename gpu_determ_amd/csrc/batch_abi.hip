@@ -372,9 +372,9 @@ int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node
         // order of the moment sums), so results do not change
         drop_graph(b);
         float best = 1e30f;
-        int bc = 4, bn = 2048, bg = 4096;
+        int bc = 4, bn = 4096, bg = 4096;
         for (int chunk : {2, 4, 8})
-            for (int nb : {1024, 2048})
+            for (int nb : {2048, 4096})
                 for (int gb : {2048, 4096}) {
                     for (gdt_context *c : b->cls)
                         if (int rc = gdt_korobov_cell_tiles(c, chunk, nb, gb, c->ct_wide)) return rc;
