@@ -45,13 +45,14 @@
 // nodes in FP32; the batch autotuner picks 2, 4 or 8 per batch).
 //
 // What bounds them (counters at 32^3, 2^21 nodes): the gather reads each
-// node's D row about 5.7 times (once per destination it feeds, scattered over
-// the grid) and each role-0 entry's partner row of f; a tile's rows (103 MB
-// in FP32) do not stay in L2, so 79% (FP32) to 92% (FP64) of its L2 requests
-// miss (about 690 cycles each), and its FETCH_SIZE + WRITE_SIZE, scaled by a
-// copy of known size, come to about 7.8 TB/s -- the memory side's rate, for
-// any chunk or grid size.  The node kernel writes each row once and reads the
-// f tile (4.4 MB in FP32, a little over one XCD's L2) at 5.4 TB/s.
+// node's D row about 4.7 times (once per non-alpha entry, scattered over the
+// grid) and each beta entry's partner row of f; a tile's rows (103 MB in
+// FP32) do not stay in L2, and 71% (FP32) to 80% (FP64) of its L2 requests
+// miss even with the octant order.  Its FETCH_SIZE + WRITE_SIZE, scaled by a
+// copy of known size, come to 7.6-7.8 TB/s: the memory side's rate.  The
+// node kernel writes each row once and reads the f tile (4.4 MB in FP32, a
+// little over one XCD's L2) at 6.3 (FP32) to 7.5 TB/s (FP64), about half
+// its instruction issue busy.
 #include <algorithm>
 
 #include "gdt_internal.h"
@@ -234,10 +235,10 @@ __device__ __forceinline__ T ct_row(const T *__restrict__ fr, uint32_t i, int k)
 // tile.  For destination a the group walks a's alpha run, positions
 // [aptr[a], aptr[a + 1]) of the tile node order, CT_NODE_UNROLL nodes per
 // pass (their loads in flight together): it stores each node's D row, and
-// sums the run's
-// alpha terms of Q0 (coefficient -1) and of the loss (2 w f_beta, FP64 w)
-// in the CSR's term form; the sums start the gather's sums of destination a
-// (Q0 and L of the tile, L before its factor f_a).  Record loads are one
+// sums the run's alpha terms of Q0 (coefficient -1) and of the loss (2 w
+// f_beta, FP64 w) in the CSR's term form; the sums start the gather's sums
+// of destination a (Q0 and L of the tile, L before its factor f_a).  Record
+// loads are one
 // address per group; the f loads and the value stores are K contiguous
 // elements (a 64- or 128-byte row).
 // Dv holds the chunk's D values [j][q][K].
@@ -327,20 +328,23 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_node(int npts, int nn, const 
 // run, then add the tile CSR's entries in order: q += coef D (coef -1 for
 // role 0, beta) and l += 2 w f_alpha over the role-0 entries; L = f_d l.
 // FP64 products and sums; the FP64 tiles keep the single-cell path's term
-// for q (coef D, then the sum), the mixed form one FMA per term.  Each round issues
-// the loads of CtUnroll<T> entries (records, then D values and f values) before
-// adding any of them, with no branch between the loads: every entry loads a D
-// value and an f value (its partner's for role 0, the destination's own --
-// one cached row -- otherwise); entries past the destination's end are
-// clamped to its last one and add exact zeros (+-0 leaves a sum unchanged).
+// for q (coef D, then the sum), the mixed form one FMA per term.  Each round
+// issues the loads of several entries (records, then D rows and f rows)
+// before adding any of them; entries past a run's end are clamped to its
+// last one and add exact zeros (+-0 leaves a sum unchanged).  The FP64
+// gather walks the entries in (q, slot) order and loads a partner's f row on
+// the beta entries only (the role is wave-uniform); the FP32 gather reads a
+// copy of the tile CSR with each destination's beta entries first and runs
+// the beta run (D and f rows) and then the rest (D rows only).
 // Destination order: with a grid of a multiple of 8 blocks (workgroups are
 // dealt to the XCDs round robin), block b takes its slots from XCD b % 8's
 // list, one octant of the box in Morton order (xperm / xo), else from the
 // longest-first order dperm.  A node's row is read by lambda and mu and by
 // their grid neighbours lambda + s and mu - s, so on one XCD, working
 // through a compact part of the grid, the second read often finds the row
-// in L2: FP64 gather 2.475 -> 2.224 us per cell against longest first; the
-// FP32 gather is unchanged (1.33), its time is not set by the missed bytes.
+// in L2: FP64 gather 2.475 -> 2.224 us per cell against longest first (the
+// FP32 gather, then still loading an f row for every entry, was bound by its
+// L2 request count and unchanged at 1.33).
 struct CtXcd {
     int off[9];
 };
@@ -361,7 +365,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_ct_gather(int npts, int nn, cons
     const size_t to = (size_t)(t0 + blockIdx.y) * npts * K;
     // grids of a multiple of 8 blocks: block b walks XCD b % 8's octant list
     const bool xcd = (gridDim.x & 7u) == 0u;
-    const int x = blockIdx.x & 7, lo = xcd ? xo.off[x] : 0, cnt = xcd ? xo.off[x + 1] - lo : npts;
+    const int xc = blockIdx.x & 7, lo = xcd ? xo.off[xc] : 0, cnt = xcd ? xo.off[xc + 1] - lo : npts;
     const int sb = xcd ? (int)(blockIdx.x >> 3) : (int)blockIdx.x, nb = xcd ? (int)(gridDim.x >> 3) : (int)gridDim.x;
     const int32_t *perm = xcd ? xperm + lo : dperm;
     for (int s = sb * NPB + g; s < cnt; s += nb * NPB) {
