@@ -473,6 +473,17 @@ def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed
         gbs = b_ * out[name]["evals_per_s"] / 1e9
         out[name]["roofline"] = {"bound": "hbm", "alg_bytes_per_eval": int(b_), "achieved": round(gbs, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    # FP64 work of the FP64 tiles' node kernel per cell (the one-cell node
+    # kernel's operation count: 54 per node with r != 0, 8 with r = 0), over
+    # its event-timed time per cell
+    nz = (inf["entries"] - 4 * inf["nodes"]) // 2
+    node_flop = 54 * nz + 8 * (inf["nodes"] - nz)
+    us = out["plain_fp64_tiles"]["kernel_us_per_cell"].get("ct_node")
+    if us:
+        tf = node_flop / (us * 1e-6) / 1e12
+        out["plain_fp64_tiles"]["ct_node_fp64"] = {"flop_per_cell": node_flop, "tflops": round(tf, 2),
+                                                   "peak_tflops": FP64_PEAK_TFLOPS,
+                                                   "frac": round(tf / FP64_PEAK_TFLOPS, 4)}
     for name in ("plain_fp64_grouped", "plain_fp64_tiles", "mixed_fp32_tiles"):
         e = out[name]["evals_per_s"]
         out[name]["speedup_vs_plain_fp64_single"] = round(e / single_evals, 2)
