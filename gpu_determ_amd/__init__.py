@@ -291,7 +291,7 @@ class Context:
         tiles of a built stencil."""
         check(lib().gdt_korobov_set_tiles(self.h, int(tiles_max)))
 
-    def korobov_cell_tiles(self, chunk_tiles=8, node_blocks=4096, gather_blocks=4096, wide=1):
+    def korobov_cell_tiles(self, chunk_tiles=16, node_blocks=4096, gather_blocks=4096, wide=1):
         """Launch shape of the cell-tile paths (chunk of tiles, grid sizes,
         tile width 16 / 32 or 64 cells); the defaults are the library's
         own, which a new context starts with."""

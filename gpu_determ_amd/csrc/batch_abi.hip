@@ -373,7 +373,7 @@ int gdt_batch_tune(gdt_batch_handle b, unsigned flags, int *seg_cells, int *node
         drop_graph(b);
         float best = 1e30f;
         int bc = 4, bn = 4096, bg = 4096;
-        for (int chunk : {2, 4, 8})
+        for (int chunk : {4, 8, 16})
             for (int nb : {2048, 4096})
                 for (int gb : {2048, 4096}) {
                     for (gdt_context *c : b->cls)

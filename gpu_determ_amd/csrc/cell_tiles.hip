@@ -574,17 +574,26 @@ int ct_cells_per_tile(const gdt_context *h, bool mixed)
     return h->ct_wide ? WAVE : (mixed ? CtK<float>::K : CtK<double>::K);
 }
 
+// Tiles of one node / gather launch: the configured chunk, at most the tiles
+// there are, and at most what keeps a chunk's node terms within 32-bit byte
+// offsets (the narrow tiles index them so).
+static long ct_chunk(const gdt_context *h, long tiles, size_t sz, int K)
+{
+    const long fit = (long)(4294967295.0 / ((double)sz * (double)h->nn * K));
+    return std::max(1L, std::min({tiles, (long)h->ct_chunk_tiles, fit}));
+}
+
 int ct_prepare(gdt_context *h, int cells, bool mixed)
 {
     const int K = ct_cells_per_tile(h, mixed);
     const size_t sz = mixed ? sizeof(float) : sizeof(double);
     const long tiles = (cells + K - 1) / K;
-    const long chunk = std::min<long>(tiles, h->ct_chunk_tiles);
+    const long chunk = ct_chunk(h, tiles, sz, K);
     const size_t np = h->n_pnt;
     GDT_CHECK_GRID_Y(tiles, "cell tiles: tiles of cells");
-    // node values of a chunk and an f tile are indexed with 32-bit byte offsets
-    GDT_CHECK((double)sz * chunk * h->nn * K < 4294967296.0 && h->nn < (1L << 30), GDT_E_ARG,
-              "cell tiles: %ld node values per chunk exceed the 32-bit offsets", chunk * h->nn * K);
+    // one tile's node values are indexed with 32-bit byte offsets
+    GDT_CHECK((double)sz * h->nn * K < 4294967296.0 && h->nn < (1L << 30), GDT_E_ARG,
+              "cell tiles: %ld node values per tile exceed the 32-bit offsets", h->nn * K);
     GDT_CHECK(h->ct_f.ensure(sz * tiles * np * K) == 0 && h->ct_D.ensure(sz * chunk * h->nn * K) == 0 &&
                   h->ct_Q0.ensure(8 * tiles * np * K) == 0 &&
                   h->ct_L.ensure(8 * tiles * np * K) == 0 &&
@@ -599,7 +608,7 @@ static int ct_launch(gdt_context *h, const double *f, int cells, bool correct, d
                      double *nu_out, double dt, double *f_upd)
 {
     const int np = h->n_pnt, tiles = (cells + K - 1) / K;
-    const int chunk = std::max(1, std::min(tiles, h->ct_chunk_tiles));
+    const int chunk = (int)ct_chunk(h, tiles, sizeof(T), K);
     T *ft = h->ct_f.as<T>(), *Dv = h->ct_D.as<T>();
     double *Q0 = h->ct_Q0.as<double>(), *L = h->ct_L.as<double>();
     GDT_LAUNCH_TIMED(h, "ct_pack", (k_ct_pack<T, K>), dim3((np + 63) / 64, tiles), dim3(CT_THREADS), 0u, np, cells,

@@ -154,7 +154,7 @@ struct gdt_context {
     // cell-tile scratch: f tiles, node values D of a chunk of tiles, per-tile
     // Q0 / L rows, moment partials, correction coefficients
     DevBuf ct_f, ct_D, ct_Q0, ct_L, ct_mp, ct_X;
-    int ct_chunk_tiles = 8;      // tiles whose node values are in flight between node and gather
+    int ct_chunk_tiles = 16;     // tiles whose node values are in flight between node and gather
     int ct_node_blocks = 4096;   // grid of the node kernel per tile (grid-stride over destinations)
     int ct_gather_blocks = 4096; // grid of the gather per tile (grid-stride over destinations)
     int ct_wide = 1;             // 1: tiles of 64 cells (a wave per node / destination)

@@ -156,7 +156,8 @@ int gdt_korobov_set_tiles(gdt_handle h, int tiles_max);
  * destination's sums follow the CSR order whatever the shape, so results are
  * the same bits for every chunk and grid size; the tile width changes the
  * order of the moment sums, so the corrected Q in its last bits.  Defaults:
- * 8 tiles, node grid 4096, gather grid 4096, wide. */
+ * 16 tiles (fewer where a chunk's node values would pass 4 GiB), node grid
+ * 4096, gather grid 4096, wide. */
 int gdt_korobov_cell_tiles(gdt_handle h, int chunk_tiles, int node_blocks, int gather_blocks, int wide);
 /* Accuracy hook, no reference counterpart: out[i] = x[i]^r[i] as the node
  * terms compute it (table-driven FP64 log and exp); device pointers, x > 0;
@@ -305,7 +306,7 @@ int gdt_batch_config(gdt_batch_handle b, int seg_cells, int node_q);
  * gdt_korobov_set_tiles, so single runs of those contexts keep matching the
  * batch bitwise), then every (segment size, nodes per thread) candidate, and
  * keeps the fastest.  With GDT_FLAG_CELL_TILES or GDT_FLAG_MIXED in flags it
- * tunes the cell tiles instead: chunk of tiles (2, 4, 8) x node grid (2048,
+ * tunes the cell tiles instead: chunk of tiles (4, 8, 16) x node grid (2048,
  * 4096) x gather grid (2048, 4096), set on every class as
  * gdt_korobov_cell_tiles does with the classes' own tile width (results do
  * not change); *seg_cells then returns the cells per chunk and *node_q the
