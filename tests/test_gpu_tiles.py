@@ -203,3 +203,36 @@ def test_cfg1_bimaxwellian_all_forms():
         say("  %s: %.3g" % (name, err))
         assert err < tol
         conserved(g.xi, Qh, g.d3v)
+
+
+def test_tiles_chunk_cut_to_32bit_offsets():
+    """At 32^3 with 2^22 lattice nodes one tile's FP64 node terms take about
+    413 MB, so the default chunk of 16 tiles would pass the 4 GiB that the
+    kernels' 32-bit offsets reach: the chunk is cut to 10 tiles.  11 tiles
+    (704 cells) then run as chunks of 10 and 1, with the same bits as chunks
+    of 2 tiles; two of the cells against the oracle at 1e-12.  The mixed form
+    (207 MB per tile) keeps whole chunks."""
+    say("cell tiles, chunk cut at 2^22")
+    g, c, st = setup(32, 1 << 22, 11)
+    nn = c.korobov_info()["nodes"]
+    assert 10 == (2 ** 32 - 1) // (8 * nn * 64) < 16 <= (2 ** 32 - 1) // (4 * nn * 64)
+    rng = np.random.default_rng(400)
+    cells = 704
+    f = np.stack([mixture(g.xi, rng) for _ in range(cells)])
+    fd = G.DeviceArray.from_host(f)
+    out = {}
+    for fl in (G.FLAG_CELL_TILES, G.FLAG_MIXED):
+        c.korobov_cell_tiles()
+        _, _, Q = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_CORRECT | fl)
+        c.korobov_cell_tiles(2)
+        _, _, Q2 = c.evaluate(G.METHOD_KOROBOV, fd, cells, G.FLAG_CORRECT | fl)
+        out[fl] = Q.get()
+        assert np.array_equal(out[fl], Q2.get())
+    c.korobov_cell_tiles()
+    pick = [0, cells - 1]
+    _, _, qc = oracle_eval(g, st, f[pick])
+    err = rel_l2(out[G.FLAG_CELL_TILES][pick], qc)
+    say("  FP64 tiles vs oracle %.3g" % err)
+    assert err < 1e-12
+    assert max(rel_l2(out[G.FLAG_MIXED][i], qc[k]) for k, i in enumerate(pick)) <= 1e-4
+    conserved(g.xi, out[G.FLAG_CELL_TILES][pick], g.d3v)
