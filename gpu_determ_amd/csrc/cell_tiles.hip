@@ -41,18 +41,19 @@
 // No atomics: every sum has a fixed order, so results are reproducible run to
 // run and do not depend on the launch shape or on the other cells of a tile.
 // Chunks of tiles bound the node values in flight between the node and
-// gather kernels (8 tiles of 64 cells by default: 825 MB at 32^3 and 2^21
-// nodes in FP32; the batch autotuner picks 2, 4 or 8 per batch).
+// gather kernels (16 tiles of 64 cells by default: 1.65 GB at 32^3 and 2^21
+// nodes in FP32, 3.3 GB in FP64; fewer where a chunk would pass the 4 GiB
+// that 32-bit offsets reach; the batch autotuner picks 4, 8 or 16).
 //
 // What bounds them (counters at 32^3, 2^21 nodes): the gather reads each
 // node's D row about 4.7 times (once per non-alpha entry, scattered over the
 // grid) and each beta entry's partner row of f; a tile's rows (103 MB in
 // FP32) do not stay in L2, and 71% (FP32) to 80% (FP64) of its L2 requests
 // miss even with the octant order.  Its FETCH_SIZE + WRITE_SIZE, scaled by a
-// copy of known size, come to 7.6-7.8 TB/s: the memory side's rate.  The
-// node kernel writes each row once and reads the f tile (4.4 MB in FP32, a
-// little over one XCD's L2) at 6.3 (FP32) to 7.5 TB/s (FP64), about half
-// its instruction issue busy.
+// copy of known size, come to 7.8 TB/s: the memory side's rate.  The node
+// kernel writes each row once and reads the f tile (4.4 MB in FP32, a little
+// over one XCD's L2) at 6.3 (FP32) to 7.35 TB/s (FP64), about half its
+// instruction issue busy.
 #include <algorithm>
 
 #include "gdt_internal.h"
