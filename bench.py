@@ -130,16 +130,17 @@ def uneven_batch(G, cells=10000, steps=3, seed=SEED):
                        "tuned": {"cells_per_chunk": tk, "gather_blocks": tg, "best_eval_ms": round(tms, 3)}}
     # roofline per the algorithmic bytes of one evaluation: f in + Q out (16 N)
     # plus the class stencil over the cells sharing one pass of it -- one
-    # cell for the work lists (16-byte node records, 8-byte entries), 64
-    # cells for the tiles (node records and FP64 (r, w) 32 B per node, plus
+    # cell for the work lists (16-byte node records, 8-byte entries of the
+    # group CSRs, the first lambda piece of each chunk of 16 nodes), 64 cells
+    # for the tiles (node records and FP64 (r, w) 32 B per node, plus
     # the FP32 (r, w) of the mixed form; 16-byte entries of the tile CSR,
     # every entry but each node's alpha one)
     alg_wl = alg_t = alg_m = 0.0
     for i, c in enumerate(ctxs):
         k = int(np.sum(pick == i))
-        inf = c.korobov_info()
+        inf, gi = c.korobov_info(), c.korobov_gather_info()
         tent = inf["entries"] - inf["nodes"]
-        alg_wl += k * (16.0 * c.n_pnt + 16.0 * inf["nodes"] + 8.0 * inf["entries"])
+        alg_wl += k * (16.0 * c.n_pnt + 16.0 * inf["nodes"] + 8.0 * gi["entries"] + 4.0 * ((inf["nodes"] + 15) // 16))
         alg_t += k * (16.0 * c.n_pnt + (32.0 * inf["nodes"] + 16.0 * tent) / 64.0)
         alg_m += k * (16.0 * c.n_pnt + (40.0 * inf["nodes"] + 16.0 * tent) / 64.0)
 
@@ -460,13 +461,14 @@ def forms_32cube(G, ctx, single_evals, cells=2048, steps=3, oracle_cells=4, seed
     out["cells_near_equilibrium"] = int((~neq).sum())
     # algorithmic bytes of one evaluation: f in and f out (16 N) plus the
     # stencil over the cells that share one pass of it -- one cell for the
-    # grouped kernels (16-byte node records, 8-byte entries), 64 for the
+    # grouped kernels (16-byte node records, 8-byte entries of the group
+    # CSRs, the first lambda piece of each chunk of 16 nodes), 64 for the
     # tiles (node records and FP64 (r, w), 32 B per node, plus the FP32
     # (r, w) of the mixed form; 16-byte entries of the tile CSR, which holds
     # every entry but the alpha one of each node: the node kernel sums those)
-    inf = ctx.korobov_info()
+    inf, gi = ctx.korobov_info(), ctx.korobov_gather_info()
     tent = inf["entries"] - inf["nodes"]
-    alg = {"plain_fp64_grouped": 16.0 * n + 16.0 * inf["nodes"] + 8.0 * inf["entries"],
+    alg = {"plain_fp64_grouped": 16.0 * n + 16.0 * inf["nodes"] + 8.0 * gi["entries"] + 4.0 * ((inf["nodes"] + 15) // 16),
            "plain_fp64_tiles": 16.0 * n + (32.0 * inf["nodes"] + 16.0 * tent) / 64.0,
            "mixed_fp32_tiles": 16.0 * n + (40.0 * inf["nodes"] + 16.0 * tent) / 64.0}
     for name, b_ in alg.items():
@@ -662,13 +664,16 @@ def main():
     tl = ctx.korobov_tiles()
     G_ = tl["groups"]              # node groups of the gather (one per XCD)
     nparts = G_ * tl["tiles"]      # gather tiles per cell: one moment partial each
+    gi = ctx.korobov_gather_info()
+    neg, npc = gi["entries"], gi["pieces"]  # the gather's entries; lambda pieces
     alg = {
-        # node records (one uint4 each) + f in; (D, 2wP) per node out
-        "kor_node": nn * 16 + 8 * n + nn * 16,
-        # entry records (node|role, num|den) + node values (each read at least
-        # once) + group row pointers + centres in; group partials of Q and
-        # loss + moment sums out
-        "kor_gather": ne * 8 + nn * 16 + G_ * 4 * (n + 1) + 24 * n + G_ * 16 * n + nparts * 160,
+        # node records (one uint4 each) + f + the chunks' first pieces in;
+        # (D, 2wP) per node and one value per lambda piece out
+        "kor_node": nn * 16 + 8 * n + 4 * ((nn + 15) // 16) + nn * 16 + npc * 16,
+        # entry records (node|role, num|den) + node values and piece sums
+        # (each read at least once) + group row pointers + centres in; group
+        # partials of Q and loss + moment sums out
+        "kor_gather": neg * 8 + (nn + npc) * 16 + G_ * 4 * (n + 1) + 24 * n + G_ * 16 * n + nparts * 160,
         # moment sums + centres + f + group partials in; f out
         "kor_apply": nparts * 160 + 24 * n + 8 * n + G_ * 16 * n + 8 * n,
         # alternative fused kernel: 20-byte entry records + f + ptr + centres

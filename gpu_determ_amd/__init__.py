@@ -48,7 +48,8 @@ ABI_SYMBOLS = (
     "gdt_gemm_nt", "gdt_split_f64", "gdt_matrix_mfma",
     "gdt_surrogate_create", "gdt_surrogate_destroy", "gdt_surrogate_train", "gdt_surrogate_predict",
     "gdt_surrogate_infer", "gdt_surrogate_residuals", "gdt_surrogate_export", "gdt_surrogate_info",
-    "gdt_init_mixture", "gdt_korobov_tiles", "gdt_korobov_set_tiles", "gdt_kmath_pow", "gdt_kmath_tables",
+    "gdt_init_mixture", "gdt_korobov_tiles", "gdt_korobov_gather_info", "gdt_korobov_set_tiles", "gdt_kmath_pow",
+    "gdt_kmath_tables",
     "gdt_transport", "gdt_korobov_cell_tiles", "gdt_evaluate_noalloc", "gdt_surrogate_set_partner",
 )
 
@@ -133,6 +134,7 @@ def lib():
         "gdt_surrogate_info": [vp, ip, ip, ip, C.POINTER(C.c_long)],
         "gdt_init_mixture": [vp, vp, C.c_int, C.c_long, vp],
         "gdt_korobov_tiles": [vp, ip, ip, ip],
+        "gdt_korobov_gather_info": [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)],
         "gdt_korobov_set_tiles": [vp, C.c_int],
         "gdt_korobov_cell_tiles": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "gdt_transport": [vp, vp, vp, C.c_int, C.c_int, C.c_int, dp, C.c_double, ip],
@@ -285,6 +287,13 @@ class Context:
         g, t, cap = C.c_int(), C.c_int(), C.c_int()
         check(lib().gdt_korobov_tiles(self.h, C.byref(g), C.byref(t), C.byref(cap)))
         return {"groups": g.value, "tiles": t.value, "cap": cap.value}
+
+    def korobov_gather_info(self):
+        """Entries of the grouped gather's CSRs and the lambda pieces the
+        node kernel sums in place of the runs' lambda entries."""
+        e, p = C.c_long(), C.c_long()
+        check(lib().gdt_korobov_gather_info(self.h, C.byref(e), C.byref(p)))
+        return {"entries": e.value, "pieces": p.value}
 
     def korobov_set_tiles(self, tiles_max):
         """Limit of gather tiles per node group (default 64); rebuilds the

@@ -110,7 +110,7 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
         for (int s = 0; s < ncls; s++) {
             const gdt_context *h = b->cls[s];
             const size_t n = (size_t)nslot[s];
-            oV[s] = take(n * h->nn * sizeof(double2));
+            oV[s] = take(n * h->nv * sizeof(double2));
             oL[s] = take(n * KOR_NGROUPS * h->n_pnt * sizeof(double));
             oQ[s] = take(n * KOR_NGROUPS * h->n_pnt * sizeof(double));
             oM[s] = take(n * 20 * KOR_NGROUPS * h->kntile * sizeof(double));
@@ -173,6 +173,8 @@ static int batch_configure(gdt_batch *b, int seg_cells, int node_q)
             C.mp = (double *)(uintptr_t)oM[s];
             C.node_blocks = node_blocks[s];
             C.pad = 0;
+            C.nv = h->nv;
+            C.cbase = h->kcbase.as<int32_t>();
         }
     }
     GDT_CHECK(b->scratch.ensure(scratch_max) == 0, GDT_E_NOMEM, "gdt_batch: scratch allocation failed (%zu bytes)",

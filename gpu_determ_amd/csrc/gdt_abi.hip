@@ -326,7 +326,7 @@ int gdt_destroy(gdt_handle h)
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     DevBuf *bufs[] = {&h->box2c, &h->cidx,   &h->xi,     &h->cmp2box, &h->tb,     &h->ta,    &h->gptr,  &h->gbox,
                       &h->gval,  &h->kidx,   &h->kr,     &h->kw,      &h->knode,  &h->kptr,  &h->kent,  &h->kn4,
-                      &h->ke4,    &h->ke1,    &h->kwb,     &h->kgptr,  &h->kgent, &h->ktile, &h->kmath,
+                      &h->ke4,    &h->ke1,    &h->kwb,     &h->kgptr,  &h->kgent, &h->kcbase, &h->ktile, &h->kmath,
                       &h->mx_bh, &h->mx_bl,  &h->mx_bf,  &h->mx_ah,   &h->mx_al,  &h->mx_af, &h->s_mx,  &h->s_fbox,
                       &h->s_out1, &h->s_out2, &h->s_vals, &h->s_loss, &h->s_Q,   &h->s_nu,  &h->s_tmp, &h->s_dt,
                       &h->s_mpart, &h->s_part};
@@ -657,19 +657,44 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
         GDT_HIP(hipMemcpy(h->ke4.p, e4.data(), sizeof(uint32_t) * 4 * ne, hipMemcpyHostToDevice));
         GDT_HIP(hipMemcpy(h->ke1.p, e1.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice));
         // per node-group CSRs of the gather over the sorted node positions p:
-        // group of p is p * KOR_NGROUPS / nn (contiguous ranges); within a
-        // group the entries are sorted by destination, then by (p, slot)
+        // group g is [kor_group_begin(g), kor_group_begin(g + 1)) (the node
+        // kernel's XCD-aligned ranges follow the same rule); within a group
+        // the entries are sorted by destination, then by (p, slot).  Slot 2
+        // (lambda) is not an entry of its own: the node kernel forms the
+        // prefix sums of the lambda terms over each chunk of KOR_CHUNK
+        // positions, and a lambda piece -- a run of one lambda destination
+        // within a chunk -- is read as two values: +prefix at its last
+        // position and, unless it starts the chunk, -prefix before its first
+        // position.  Each such position carries an entry in slot 2 per value
+        // (first position's before last position's): value nn + (value slots
+        // before it), role 1, r = 0.
         const int G = KOR_NGROUPS;
         std::vector<int32_t> gp((size_t)G * (np + 1), 0);
-        auto grp = [&](uint32_t p) { return (int)(((uint64_t)p * G) / (uint64_t)nn); };
-        for (int g = 0; g < G; g++) {  // the node kernel's XCD-aligned ranges follow the same rule
-            const long lo = kor_group_begin(g, nn), hi = kor_group_begin(g + 1, nn);
-            GDT_CHECK(lo >= hi || (grp((uint32_t)lo) == g && grp((uint32_t)(hi - 1)) == g), GDT_E_STATE,
-                      "gdt_korobov_build: group %d range [%ld, %ld) off the group rule", g, lo, hi);
+        std::vector<int> pgrp(nn);
+        for (int g = 0; g < G; g++)
+            for (long p = kor_group_begin(g, nn); p < kor_group_begin(g + 1, nn); p++) pgrp[p] = g;
+        auto grp = [&](long p) { return pgrp[p]; };
+        // value slots: a piece's first position (not a chunk's first) and its
+        // last position; the chunks' first slots and the slots' count
+        auto lam_at = [&](long p) { return cidx6[6 * (size_t)order[p] + 2]; };
+        std::vector<uint8_t> vslot(nn);  // bit 0: first position, bit 1: last position
+        const long nchunk = (nn + KOR_CHUNK - 1) / KOR_CHUNK;
+        std::vector<int32_t> cbase(nchunk + 1);
+        long nslot = 0;
+        for (long p = 0; p < nn; p++) {
+            if (p % KOR_CHUNK == 0) cbase[p / KOR_CHUNK] = (int32_t)nslot;
+            const bool head = p % KOR_CHUNK != 0 && lam_at(p - 1) != lam_at(p);
+            const bool tail = p + 1 == nn || (p + 1) % KOR_CHUNK == 0 || lam_at(p + 1) != lam_at(p);
+            vslot[p] = (uint8_t)((head ? 1 : 0) | (tail ? 2 : 0));
+            nslot += (head ? 1 : 0) + (tail ? 1 : 0);
         }
+        cbase[nchunk] = (int32_t)nslot;
+        GDT_CHECK(nn + nslot < (1L << 30), GDT_E_ARG, "gdt_korobov_build: %ld node values exceed the entry format",
+                  nn + nslot);
+        auto nent_at = [&](long p, int sl) { return sl != 2 ? 1 : (vslot[p] & 1) + (vslot[p] >> 1); };
         for (long p = 0; p < nn; p++) {
             const long t = order[p];
-            for (int sl = 0; sl < slots(t); sl++) gp[(size_t)grp(p) * (np + 1) + cidx6[6 * t + sl] + 1]++;
+            for (int sl = 0; sl < slots(t); sl++) gp[(size_t)grp(p) * (np + 1) + cidx6[6 * t + sl] + 1] += nent_at(p, sl);
         }
         long run = 0;
         for (int g = 0; g < G; g++) {
@@ -680,19 +705,33 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
                 pg[v + 1] = (int32_t)run;
             }
         }
-        // entry records (p << 2 | role, the last word of the node's record)
-        std::vector<uint32_t> gent(2 * (size_t)ne);
+        // entry records (p << 2 | role, the last word of the node's record;
+        // a lambda value slot: (nn + slot) << 2 | 1, word 0, so r = 0)
+        const long neg = gp[(size_t)G * (np + 1) - 1];
+        std::vector<uint32_t> gent(2 * (size_t)std::max(neg, 1L));
         std::vector<int32_t> fill(gp);
+        long slot = 0;
         for (long p = 0; p < nn; p++) {
             const long t = order[p];
             const int g = grp(p);
             for (int sl = 0; sl < slots(t); sl++) {
                 const uint32_t role = sl < 2 ? 0u : (sl < 4 ? 1u : 2u);
-                const int32_t pos = fill[(size_t)g * (np + 1) + cidx6[6 * t + sl]]++;
-                gent[2 * (size_t)pos] = ((uint32_t)p << 2) | role;
-                gent[2 * (size_t)pos + 1] = n4[4 * (size_t)p + 3];
+                int32_t *fp = &fill[(size_t)g * (np + 1) + cidx6[6 * t + sl]];
+                if (sl != 2) {
+                    const int32_t pos = (*fp)++;
+                    gent[2 * (size_t)pos] = ((uint32_t)p << 2) | role;
+                    gent[2 * (size_t)pos + 1] = n4[4 * (size_t)p + 3];
+                    continue;
+                }
+                for (int b = 0; b < 2; b++)
+                    if (vslot[p] & (1 << b)) {
+                        const int32_t pos = (*fp)++;
+                        gent[2 * (size_t)pos] = ((uint32_t)(nn + slot++) << 2) | 1u;
+                        gent[2 * (size_t)pos + 1] = 0u;
+                    }
             }
         }
+        GDT_CHECK(slot == nslot, GDT_E_STATE, "gdt_korobov_build: lambda value slot count");
         // the cell-tile path's own node order: sorted by the alpha
         // destination (then lambda, then lambda+s, lattice order among
         // equals), so that every destination's alpha entries are one
@@ -837,10 +876,13 @@ int gdt_korobov_build(gdt_handle h, uint32_t M, uint64_t seed, uint32_t a, doubl
         }
         h->h_gp.swap(gp);
         GDT_CHECK(h->kgptr.ensure(sizeof(int32_t) * h->h_gp.size()) == 0 &&
-                      h->kgent.ensure(sizeof(uint32_t) * 2 * ne) == 0,
+                      h->kgent.ensure(sizeof(uint32_t) * gent.size()) == 0 &&
+                      h->kcbase.ensure(sizeof(int32_t) * cbase.size()) == 0,
                   GDT_E_NOMEM, "gdt_korobov_build: group CSR allocation failed");
         GDT_HIP(hipMemcpy(h->kgptr.p, h->h_gp.data(), sizeof(int32_t) * h->h_gp.size(), hipMemcpyHostToDevice));
-        GDT_HIP(hipMemcpy(h->kgent.p, gent.data(), sizeof(uint32_t) * 2 * ne, hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kgent.p, gent.data(), sizeof(uint32_t) * gent.size(), hipMemcpyHostToDevice));
+        GDT_HIP(hipMemcpy(h->kcbase.p, cbase.data(), sizeof(int32_t) * cbase.size(), hipMemcpyHostToDevice));
+        h->nv = nn + nslot;
         int rc1 = kor_make_tiles(h);
         if (rc1) return rc1;
         int rc2 = kor_setup(h, 1);
@@ -939,7 +981,7 @@ static int kor_prepare(gdt_context *h, int cells)
     GDT_CHECK_GRID_Y(cells, "Korobov: cells");
     const int nb = std::max(kor_blocks(h, cells), kor_gather_parts(h));
     const size_t mp = sizeof(double) * 20 * (size_t)cells * nb;
-    const size_t dw = sizeof(double) * 2 * (size_t)h->nn * cells;
+    const size_t dw = sizeof(double) * 2 * (size_t)h->nv * cells;
     const size_t pt = sizeof(double) * 2 * (size_t)KOR_NGROUPS * h->n_pnt * cells;
     GDT_CHECK(h->s_mpart.ensure(mp) == 0 && h->s_vals.ensure(dw) == 0 && h->s_part.ensure(pt) == 0, GDT_E_NOMEM,
               "Korobov scratch allocation failed (%zu bytes of node values)", dw);
@@ -1283,6 +1325,15 @@ int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap)
     if (groups) *groups = KOR_NGROUPS;
     if (tiles) *tiles = h->kntile;
     if (cap) *cap = h->kcap;
+    return GDT_OK;
+}
+
+int gdt_korobov_gather_info(gdt_handle h, long *entries, long *pieces)
+{
+    H_CHECK(h);
+    GDT_CHECK(h->have_kor && h->have_stream, GDT_E_STATE, "gdt_korobov_gather_info: stencil not built");
+    if (entries) *entries = h->h_gp.empty() ? 0 : h->h_gp.back();
+    if (pieces) *pieces = h->nv - h->nn;
     return GDT_OK;
 }
 

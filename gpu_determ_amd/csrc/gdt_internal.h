@@ -36,6 +36,7 @@ void gdt_set_error(int code, const char *fmt, ...);
 #define KOR_TILE_CAP 4864 // entries per gather pass (76 KB of LDS at most)
 #define KOR_TILES_MAX 64  // default gather tiles per group: G * 64 blocks, two per CU, one round
 #define KOR_TILES_LIMIT 1024
+#define KOR_CHUNK 16       // node positions per lambda-piece chunk: one DPP row of the node kernel
 #define KOR_NPTS_MAX 19200 // grid nodes the packed node records and the LDS copy of f admit
 // Cells (grouped Korobov and analytic-matrix paths) or 64-cell tiles (cell
 // tiles) one launch takes: they index grid y, which admits at most 65,535.
@@ -110,6 +111,7 @@ struct gdt_context {
     uint32_t kM = 0, ka = 0;
     double ks0 = 0.0, kK = 0.0;
     long nn = 0, nent = 0;
+    long nv = 0;  // node values per cell of the grouped path: nn nodes, then the lambda pieces
     DevBuf kidx, kr, kw, knode, kptr, kent;
     // per-node records (one uint4: six 15-bit node indices, num / 4, den / 4
     // and D2) and, for the fused path, per-entry records (uint4 as the
@@ -117,6 +119,7 @@ struct gdt_context {
     bool have_stream = false;
     DevBuf kn4, ke4, ke1, kwb;
     DevBuf kgptr, kgent;          // per node-group CSRs of the gather (uint2 entry records)
+    DevBuf kcbase;                // first lambda piece of each chunk of KOR_CHUNK node positions
     DevBuf ktile;                 // gather tiles (d0, d1, e0, e1), [group][kntile]
     DevBuf kmath;                 // log / exp tables of the node terms (KMT_SIZE doubles)
     int kntile = 0, kcap = 0;     // tiles per group, LDS entries per gather pass
@@ -266,12 +269,14 @@ struct KorClassDev {
     const double *kmt;  // log / exp tables
     double *f;     // [cells][npts], updated in place by steps
     double *Qout;  // [cells][npts] or NULL
-    double2 *V;    // [slots][nn]
+    double2 *V;    // [slots][nv]
     double *Lp;    // [slots][G][npts]
     double *Qp;    // [slots][G][npts]
     double *mp;    // [slots][20][nparts]
     int node_blocks;  // node work items per cell
     int pad;
+    long nv;               // node values per cell (nodes, then lambda pieces)
+    const int32_t *cbase;  // first lambda piece per chunk of node positions
 };
 // Work item: (class, cell, block within the cell, slot)
 int launch_batch_node(hipStream_t s, const KorClassDev *cls, const int4 *items, int nitems, int blocks, int lds);
@@ -287,11 +292,14 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells);
 int kor_gather_blocks(gdt_context *h, int cells);  // blocks launched per cell (one per tile)
 int kor_gather_parts(gdt_context *h);              // moment partials per cell (one per tile)
 int launch_kor_gather(gdt_context *h, const double *DW, double *Lpart, double *Qpart, double *mpart, int cells);
-// First node position of gather group g (node p is in group p * G / nn, so
-// group g is [kor_group_begin(g), kor_group_begin(g + 1)))
+// First node position of gather group g: ceil(g nn / G) rounded up to a
+// whole chunk of KOR_CHUNK positions, at most nn (group g is
+// [kor_group_begin(g), kor_group_begin(g + 1)), whole chunks but the last)
 __host__ __device__ constexpr long kor_group_begin(int g, long nn)
 {
-    return ((long)g * nn + KOR_NGROUPS - 1) / KOR_NGROUPS;
+    return ((((long)g * nn + KOR_NGROUPS - 1) / KOR_NGROUPS + KOR_CHUNK - 1) / KOR_CHUNK * KOR_CHUNK) < nn
+               ? (((long)g * nn + KOR_NGROUPS - 1) / KOR_NGROUPS + KOR_CHUNK - 1) / KOR_CHUNK * KOR_CHUNK
+               : nn;
 }
 // LDS copy of f padded to whole 128-double chunks (the node kernels' 16-byte
 // staging writes whole wave chunks); the log / exp tables follow the padding

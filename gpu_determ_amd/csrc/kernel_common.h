@@ -79,6 +79,25 @@ __device__ __forceinline__ void seg_scan_dpp(double &q, double &l, int key)
     seg_step<DPP_ROW_BCAST31, 0xC>(q, l, key);
 }
 
+// The same scan for one value.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_step1(double &u, int key)
+{
+    const int ks = dpp_i32_or<CTRL, ROWMASK>(key, -1);
+    const double us = dpp_f64_or<CTRL, ROWMASK>(u, 0.0);
+    if (ks == key) u += us;
+}
+
+__device__ __forceinline__ void seg_scan1_dpp(double &u, int key)
+{
+    seg_step1<DPP_ROW_SHR(1), 0xF>(u, key);
+    seg_step1<DPP_ROW_SHR(2), 0xF>(u, key);
+    seg_step1<DPP_ROW_SHR(4), 0xF>(u, key);
+    seg_step1<DPP_ROW_SHR(8), 0xF>(u, key);
+    seg_step1<DPP_ROW_BCAST15, 0xA>(u, key);
+    seg_step1<DPP_ROW_BCAST31, 0xC>(u, key);
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);

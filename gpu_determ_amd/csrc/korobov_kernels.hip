@@ -10,14 +10,19 @@
 // Default path, three kernels per step:
 //   k_kor_node    node-parallel, f of the cell staged in LDS (16-byte LDS
 //                 DMA): D and 2wP of every node, once, written contiguously
-//                 (double2 per node); block b evaluates a slice of node group
-//                 b % KOR_GROUPS, on the XCD that gathers that group;
+//                 (double2 per node), and the sums of the lambda pieces (a
+//                 run of one lambda destination within a chunk of 16 node
+//                 positions; the records are sorted by lambda); block b
+//                 evaluates a slice of node group b % KOR_GROUPS, on the XCD
+//                 that gathers that group;
 //   k_kor_gather  destination-sorted reduction, XCD-aware: the nodes are
 //                 split into KOR_GROUPS contiguous ranges, each with its own
 //                 destination-sorted CSR of (node | role, coefficient -1 /
-//                 1-r / r) entries.  Workgroups are dealt to the XCDs round
-//                 robin, so block b serves group b % KOR_GROUPS and every XCD
-//                 gathers only its range's (D, 2wP) -- under 1 MB, resident in
+//                 1-r / r) entries, where one entry per lambda piece (its
+//                 sum, coefficient 1) stands for the piece's lambda entries.
+//                 Workgroups are dealt to the XCDs round robin, so block b
+//                 serves group b % KOR_GROUPS and every XCD gathers only its
+//                 range's (D, 2wP) and piece sums -- under 1 MB, resident in
 //                 that XCD's L2.  A block takes one tile of a group's
 //                 destinations: it parks every entry's term in LDS, then a
 //                 thread (or, for tiles of long rows, a quad) sums each
@@ -124,6 +129,7 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, 
 // ---------------------------------------------------------------------------
 // Per node t of cell c, the values the destination entries read:
 //   V[t] = (D, 2 w P)
+// and, after the nn nodes, one value per lambda piece (below).
 // Block b of a grid of B blocks per cell owns a contiguous node range:
 //   B a multiple of KOR_GROUPS (xcd): the range is slice b / G of the S = B / G
 //     equal slices of gather group g = b % G.  Workgroups are dealt to the
@@ -131,13 +137,32 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, 
 //     whose L2 the gather of group g then reads them through (one step:
 //     29.0 -> 26.7 us against group-blind ranges);
 //   otherwise: [b per, (b + 1) per), per = ceil(nn / B).
-// Either way every block has the same share (within one node per thread).  A
-// thread owns nodes t and t + KOR_THREADS of
+// Slices are whole waves of positions (groups start on chunk boundaries),
+// so every block has the same share within a wave and lane i of a wave
+// holds a position i mod 64.  A thread owns nodes t and t + KOR_THREADS of
 // the range and evaluates both together, so the two dependency chains (LDS
 // reads, log / exp) overlap; their records are loaded before the staging of
 // f, so that latency overlaps too.  Further nodes (more than two per thread)
 // are handled one at a time.  f of the cell is fc; values go to out (the
 // cell's V).
+//
+// Lambda pieces.  The records are sorted by the lambda destination, so a
+// destination's lambda terms (1 - r) D come from one run of positions.  The
+// positions are cut into chunks of KOR_CHUNK = 16, one DPP row of a wave
+// (lane i of a wave holds position 64 c + i), and a piece is a run of one
+// lambda within a chunk.  The row forms the prefix sums I of its terms
+// (row shifts 1, 2, 4, 8: a fixed pairing order) and a piece [s, e] is
+// I_e - I_(s-1): its last position writes +I_e and, unless it starts the
+// chunk, its first position writes -I_(s-1), each to its own value slot
+// out[nn + k], k = cbase[c] + the slots of the positions before it (a first
+// position's slot before a last position's).  Chunks are absolute
+// positions, so the values do not depend on the launch shape (the grouped,
+// multi-cell and batch grids give the same bits).  The gather reads each
+// slot as one entry (role 1 with r = 0: coefficient 1) in place of the
+// piece's lambda entries: about 58,000 slots for 403,000 lambda entries at
+// 32^3 and 2^21 nodes.  (A segmented scan per piece costs twice the
+// instructions of the prefix sums, and the node kernel's arithmetic is
+// issue-bound.)
 typedef unsigned int u32x4_nt __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint4 load_nt_u4(const uint4 *p)
 {
@@ -145,9 +170,44 @@ __device__ __forceinline__ uint4 load_nt_u4(const uint4 *p)
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
+#define DPP_ROW_SHL(k) (0x100 + (k))  // lane i of a row reads lane i + k of the row
+static_assert(KOR_CHUNK == 16, "a chunk of positions is one DPP row of 16 lanes");
+// u: (1 - r) D of the lane's position (0 where not valid); lam its lambda;
+// cb: the first value slot of the wave's first chunk (loaded by the caller
+// ahead of time: a load here would hold up the end of the wave).
+__device__ __forceinline__ void lambda_pieces(double u, uint32_t lam, bool valid, int cb, double2 *__restrict__ P)
+{
+#pragma clang fp contract(off)
+    const int lane = (int)(threadIdx.x % WAVE);
+    const int key = valid ? (int)lam : -2 - lane;
+    double I = u;
+    I += dpp_f64_or<DPP_ROW_SHR(1), 0xF>(I, 0.0);
+    I += dpp_f64_or<DPP_ROW_SHR(2), 0xF>(I, 0.0);
+    I += dpp_f64_or<DPP_ROW_SHR(4), 0xF>(I, 0.0);
+    I += dpp_f64_or<DPP_ROW_SHR(8), 0xF>(I, 0.0);
+    const double E = dpp_f64_or<DPP_ROW_SHR(1), 0xF>(I, 0.0);  // I of the previous position
+    const int kprev = dpp_i32_or<DPP_ROW_SHR(1), 0xF>(key, -1);
+    const int knext = dpp_i32_or<DPP_ROW_SHL(1), 0xF>(key, -1);
+    const bool head = valid && kprev != key && (lane & (KOR_CHUNK - 1)) != 0, tail = valid && knext != key;
+    const unsigned long long mh = __ballot(head), mt = __ballot(tail);
+    if (head || tail) {
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int k = cb + __popcll(mh & below) + __popcll(mt & below);
+        if (head) {
+            __builtin_nontemporal_store(-E, &P[k].x);
+            __builtin_nontemporal_store(0.0, &P[k].y);
+        }
+        if (tail) {
+            __builtin_nontemporal_store(I, &P[k + (head ? 1 : 0)].x);
+            __builtin_nontemporal_store(0.0, &P[k + (head ? 1 : 0)].y);
+        }
+    }
+}
+
 __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const uint4 *__restrict__ nrec, double K,
                                            const double *__restrict__ fc, double2 *__restrict__ out, int b, int B,
-                                           const double *__restrict__ kmt, bool xcd)
+                                           const double *__restrict__ kmt, bool xcd,
+                                           const int32_t *__restrict__ cbase)
 {
 #pragma clang fp contract(off)
     long lo = 0, hi = nn, k = b, S = B;
@@ -158,14 +218,18 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
         k = b / KOR_GROUPS;
         S = B / KOR_GROUPS;
     }
-    const long per = (hi - lo + S - 1) / S, t0 = lo + k * per;
+    const long per = ((hi - lo + S - 1) / S + WAVE - 1) / WAVE * WAVE, t0 = lo + k * per;
     const long tend = t0 + per < hi ? t0 + per : hi;
     const long ta = t0 + threadIdx.x, tb = ta + KOR_THREADS;
+    const long lane = threadIdx.x % WAVE;
     uint4 xa = make_uint4(0u, 0u, 0u, 0u), xb = xa;
     // streaming (nontemporal) loads: records are read once per step, and
     // should not push the gather's entries out of L2
     if (ta < tend) xa = load_nt_u4(&nrec[ta]);
     if (tb < tend) xb = load_nt_u4(&nrec[tb]);
+    // the waves' first value slots, in flight with the records
+    const int cba = ta - lane < tend ? cbase[(ta - lane) / KOR_CHUNK] : 0;
+    const int cbb = tb - lane < tend ? cbase[(tb - lane) / KOR_CHUNK] : 0;
     stage_f(fs, fc, npts, kmt);
     __syncthreads();
     const double *ts = fs + kor_fpad(npts);
@@ -182,22 +246,34 @@ __device__ __forceinline__ void node_block(double *fs, int npts, long nn, const 
         __builtin_nontemporal_store(Db, &out[tb].x);
         __builtin_nontemporal_store(Wb, &out[tb].y);
     }
-    for (long t = tb + KOR_THREADS; t < tend; t += KOR_THREADS) {
+    double2 *P = out + nn;
+    lambda_pieces(ta < tend ? (1.0 - ra) * Da : 0.0, xa.y & 0x7FFFu, ta < tend, cba, P);
+    lambda_pieces(tb < tend ? (1.0 - rb) * Db : 0.0, xb.y & 0x7FFFu, tb < tend, cbb, P);
+    // further nodes while the wave's first position is before tend (a
+    // uniform condition: every lane of the wave takes part in the sums)
+    for (long t = tb + KOR_THREADS; t - lane < tend; t += KOR_THREADS) {
+        const bool v = t < tend;
+        const uint4 x = v ? nrec[t] : make_uint4(0u, 0u, 0u, 0u);
+        const int cb = cbase[(t - lane) / KOR_CHUNK];
         double D, W2P, r;
-        node_terms(nrec[t], fs, ts, K, D, W2P, r);
-        __builtin_nontemporal_store(D, &out[t].x);
-        __builtin_nontemporal_store(W2P, &out[t].y);
+        node_terms(x, fs, ts, K, D, W2P, r);
+        if (v) {
+            __builtin_nontemporal_store(D, &out[t].x);
+            __builtin_nontemporal_store(W2P, &out[t].y);
+        }
+        lambda_pieces(v ? (1.0 - r) * D : 0.0, x.y & 0x7FFFu, v, cb, P);
     }
 }
 
-__global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, const uint4 *__restrict__ nrec,
+__global__ __launch_bounds__(KOR_THREADS) void k_kor_node(int npts, long nn, long nv, const uint4 *__restrict__ nrec,
                                                            double K, const double *__restrict__ f,
-                                                           double2 *__restrict__ V, const double *__restrict__ kmt)
+                                                           double2 *__restrict__ V, const double *__restrict__ kmt,
+                                                           const int32_t *__restrict__ cbase)
 {
     extern __shared__ double fs[];
     const int c = blockIdx.y;
-    node_block(fs, npts, nn, nrec, K, f + (size_t)c * npts, V + (size_t)c * nn, blockIdx.x, gridDim.x, kmt,
-               gridDim.x % KOR_GROUPS == 0);
+    node_block(fs, npts, nn, nrec, K, f + (size_t)c * npts, V + (size_t)c * nv, blockIdx.x, gridDim.x, kmt,
+               gridDim.x % KOR_GROUPS == 0, cbase);
 }
 
 // ---------------------------------------------------------------------------
@@ -389,7 +465,7 @@ __device__ __forceinline__ void gather_block(double2 *ql, double (*rpart)[20], i
     block_moments<TILE_THREADS>(m, rpart, mp, b, nparts);
 }
 
-__global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, int ntile, int cap,
+__global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nv, int ntile, int cap,
                                                               const int4 *__restrict__ tiles,
                                                               const int32_t *__restrict__ gptr,
                                                               const uint2 *__restrict__ ent,
@@ -402,7 +478,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_kor_gather(int npts, long nn, 
     __shared__ double rpart[TILE_THREADS / 16][20];
     const int c = blockIdx.y;
     const size_t po = (size_t)c * KOR_GROUPS * npts;
-    gather_block(ql, rpart, npts, ntile, cap, tiles, gptr, ent, V + (size_t)c * nn, xi, Lpart + po, Qpart + po,
+    gather_block(ql, rpart, npts, ntile, cap, tiles, gptr, ent, V + (size_t)c * nv, xi, Lpart + po, Qpart + po,
                  mpart + (size_t)c * 20 * gridDim.x, blockIdx.x, gridDim.x);
 }
 
@@ -653,8 +729,8 @@ __global__ __launch_bounds__(KOR_THREADS) void k_batch_node(const KorClassDev *_
         if (i != (int)blockIdx.x) __syncthreads();  // the previous item's LDS reads are done
         const int4 it = items[i];
         const KorClassDev C = cls[it.x];
-        node_block(fs, C.npts, C.nn, C.nrec, C.K, C.f + (size_t)it.y * C.npts, C.V + (size_t)it.w * C.nn,
-                   it.z, C.node_blocks, C.kmt, false);
+        node_block(fs, C.npts, C.nn, C.nrec, C.K, C.f + (size_t)it.y * C.npts, C.V + (size_t)it.w * C.nv,
+                   it.z, C.node_blocks, C.kmt, false, C.cbase);
     }
 }
 
@@ -668,7 +744,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_batch_gather(const KorClassDev
         const int4 it = items[i];
         const KorClassDev C = cls[it.x];
         const size_t po = (size_t)it.w * KOR_GROUPS * C.npts;
-        gather_block(ql, rpart, C.npts, C.ntile, C.cap, C.tiles, C.gptr, C.ent, C.V + (size_t)it.w * C.nn, C.xi,
+        gather_block(ql, rpart, C.npts, C.ntile, C.cap, C.tiles, C.gptr, C.ent, C.V + (size_t)it.w * C.nv, C.xi,
                      C.Lp + po, C.Qp + po, C.mp + (size_t)it.w * 20 * C.nparts, it.z, C.nparts);
     }
 }
@@ -805,7 +881,8 @@ int launch_kor_node(gdt_context *h, const double *f, double *DW, int cells)
     GDT_CHECK_GRID_Y(cells, "kor_node: cells");
     const int B = kor_blocks(h, cells);
     GDT_LAUNCH_TIMED(h, "kor_node", k_kor_node, dim3(B, cells), dim3(KOR_THREADS), kor_node_lds(h), h->n_pnt, h->nn,
-                     h->kn4.as<uint4>(), h->kK, f, (double2 *)DW, h->kmath.as<double>());
+                     h->nv, h->kn4.as<uint4>(), h->kK, f, (double2 *)DW, h->kmath.as<double>(),
+                     h->kcbase.as<int32_t>());
     GDT_HIP(hipGetLastError());
     return GDT_OK;
 }
@@ -824,7 +901,7 @@ int launch_kor_gather(gdt_context *h, const double *V, double *Lpart, double *Qp
     GDT_CHECK_GRID_Y(cells, "kor_gather: cells");
     const int B = kor_gather_blocks(h, cells);
     GDT_LAUNCH_TIMED(h, "kor_gather", k_kor_gather, dim3(B, cells), dim3(TILE_THREADS),
-                     (uint32_t)(sizeof(double2) * h->kcap), h->n_pnt, h->nn, h->kntile, h->kcap,
+                     (uint32_t)(sizeof(double2) * h->kcap), h->n_pnt, h->nv, h->kntile, h->kcap,
                      h->ktile.as<int4>(), h->kgptr.as<int32_t>(), h->kgent.as<uint2>(), (const double2 *)V,
                      h->xi.as<double>(), Lpart, Qpart, mpart);
     GDT_HIP(hipGetLastError());

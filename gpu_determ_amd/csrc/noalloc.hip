@@ -56,13 +56,13 @@ extern "C" int gdt_evaluate_noalloc(gdt_handle h, const double *f, double *nu, d
         base = h->device >= 0 && h->device < NOALLOC_MAX_DEVICES ? g_scratch_ptr[h->device] : nullptr;
     }
     GDT_CHECK(base, GDT_E_STATE, "gdt_evaluate_noalloc: no scratch address for device %d", h->device);
-    // node values (D, 2wP) per node and cell; group partials of Q and loss;
-    // moment sums per gather tile
+    // node values (D, 2wP) per node and the lambda pieces, per cell; group
+    // partials of Q and loss; moment sums per gather tile
     const int nparts = kor_gather_parts(h);
-    const size_t vb = align256(sizeof(double) * 2 * (size_t)h->nn * cells);
+    const size_t vb = align256(sizeof(double) * 2 * (size_t)h->nv * cells);
     const size_t pb = align256(sizeof(double) * 2 * (size_t)KOR_NGROUPS * h->n_pnt * cells);
     const size_t mb = align256(sizeof(double) * 20 * (size_t)nparts * cells);
-    const size_t per_cell = align256(sizeof(double) * 2 * (size_t)h->nn) +
+    const size_t per_cell = align256(sizeof(double) * 2 * (size_t)h->nv) +
                             align256(sizeof(double) * 2 * (size_t)KOR_NGROUPS * h->n_pnt) +
                             align256(sizeof(double) * 20 * (size_t)nparts);
     GDT_CHECK(vb + pb + mb <= NOALLOC_SCRATCH_BYTES, GDT_E_ARG,

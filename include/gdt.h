@@ -137,6 +137,13 @@ int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32
 /* Gather layout of the stencil: node groups (one per XCD), destination
  * tiles per group (one moment partial each), LDS entries per gather pass. */
 int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
+/* Sizes of the grouped gather; no reference counterpart (layout figures for
+ * byte counts).  The node kernel sums each lambda piece -- a run of one
+ * lambda destination within a chunk of 16 node positions -- and the gather
+ * reads the piece as one entry in place of the run's lambda entries:
+ * *entries is the entries of the node groups' CSRs, *pieces the pieces (node
+ * values per cell: nodes + pieces). */
+int gdt_korobov_gather_info(gdt_handle h, long *entries, long *pieces);
 /* Limit of gather tiles per node group (default 64, 1..1024); no reference
  * counterpart (a launch-shape setting of the gather).  A tile holds at most
  * 512 destinations, so a group has at least ceil(grid nodes / 512) tiles
@@ -205,8 +212,8 @@ int gdt_evaluate(gdt_handle h, int method, const double *f, double *nu, double *
  * GDT_METHOD_KOROBOV, f, nu, gain, Q, cells, GDT_LAYOUT_CELL_MAJOR, flags),
  * bit for bit; flags is 0 or GDT_FLAG_CORRECT; nu and gain may be NULL (gain
  * needs nu).  Its scratch (node values, group partials, moment sums) is one
- * static 64 MiB device array per GPU: calls needing more (more than about 7
- * cells of a 32^3 grid with 2^21 lattice nodes) return GDT_E_ARG.  Every
+ * static 64 MiB device array per GPU: calls needing more (more than 6 cells
+ * of a 32^3 grid with 2^21 lattice nodes) return GDT_E_ARG.  Every
  * call on a GPU shares that array, so calls that overlap on one GPU race.
  * Calls no hipMalloc or hipFree; asynchronous on the handle's stream. */
 int gdt_evaluate_noalloc(gdt_handle h, const double *f, double *nu, double *gain, double *Q, int cells,

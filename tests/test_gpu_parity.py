@@ -235,6 +235,34 @@ def test_korobov_collide_matches_oracle(cells):
     assert np.array_equal(l2.get().reshape(g.n_pnt, cells).T, loss.get())
 
 
+def test_korobov_bits_do_not_depend_on_cells_per_call():
+    """The node kernel splits the nodes by the launch shape (256 blocks per
+    cell for one cell, 88 for 3, 7 for 40, 1 for 300), and sums each lambda
+    piece -- a run of one lambda destination within an absolute chunk of 16
+    node positions -- whatever the split.  So every cell's loss and
+    corrected Q are the same bits however many cells share the call."""
+    say("korobov bits per cells per call")
+    vm = 4.8
+    g = O.Grid((20, 20, 20), (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    c.korobov_build(1 << 16, seed=5, s0=0.25)
+    gi = c.korobov_gather_info()
+    assert 0 < gi["pieces"] < c.korobov_info()["nodes"]
+    rng = np.random.default_rng(21)
+    f = np.stack([mixture(g.xi, rng) for _ in range(300)])
+    fd = G.DeviceArray.from_host(f)
+    loss, Q = c.collide_korobov(fd, 300)
+    _, _, Qc = c.evaluate(G.METHOD_KOROBOV, fd, 300, G.FLAG_CORRECT)
+    L300, Q300, Qc300 = loss.get(), Q.get(), Qc.get()
+    for cells in (1, 3, 40):
+        part = G.DeviceArray.from_host(f[:cells])
+        l, q = c.collide_korobov(part, cells)
+        _, _, qc = c.evaluate(G.METHOD_KOROBOV, part, cells, G.FLAG_CORRECT)
+        assert np.array_equal(l.get(), L300[:cells])
+        assert np.array_equal(q.get(), Q300[:cells])
+        assert np.array_equal(qc.get(), Qc300[:cells])
+
+
 class _DeviceView:
     """A device pointer into another DeviceArray (which keeps ownership)."""
 
@@ -441,7 +469,7 @@ def test_kmath_pow_accuracy():
 
 def test_korobov_gather_multipass():
     """Tiles with more entries than one LDS pass holds (16 tiles per group at
-    20^3 and 2^20 nodes: about 8,700 entries per tile against a pass of at most
+    20^3 and 2^20 nodes: about 7,300 entries per tile against a pass of at most
     4,864) are summed in passes; the result still matches the oracle, and the
     default tiling to FP64 rounding."""
     say("korobov gather multipass")
@@ -457,7 +485,7 @@ def test_korobov_gather_multipass():
     l1, q1 = c.collide_korobov(fd, 2)
     c.korobov_set_tiles(16)
     tl = c.korobov_tiles()
-    assert tl["tiles"] <= 16 and c.korobov_info()["entries"] / (tl["groups"] * tl["tiles"]) > tl["cap"]
+    assert tl["tiles"] <= 16 and c.korobov_gather_info()["entries"] / (tl["groups"] * tl["tiles"]) > tl["cap"]
     l2, q2 = c.collide_korobov(fd, 2)
     ol, oq = O.korobov_collide(g, st, f)
     assert rel_l2(q2.get(), oq) < 1e-12 and rel_l2(l2.get(), ol) < 1e-12
