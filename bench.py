@@ -665,9 +665,9 @@ def main():
     G_ = tl["groups"]              # node groups of the gather (one per XCD)
     nparts = G_ * tl["tiles"]      # gather tiles per cell: one moment partial each
     gi = ctx.korobov_gather_info()
-    neg, npc = gi["entries"], gi["pieces"]  # the gather's entries; lambda pieces
+    neg, npc = gi["entries"], gi["slots"]  # the gather's entries; lambda value slots
     alg = {
-        # node records (one uint4 each) + f + the chunks' first pieces in;
+        # node records (one uint4 each) + f + the chunks' first slots in;
         # (D, 2wP) per node and one value per lambda piece out
         "kor_node": nn * 16 + 8 * n + 4 * ((nn + 15) // 16) + nn * 16 + npc * 16,
         # entry records (node|role, num|den) + node values and piece sums

@@ -289,11 +289,11 @@ class Context:
         return {"groups": g.value, "tiles": t.value, "cap": cap.value}
 
     def korobov_gather_info(self):
-        """Entries of the grouped gather's CSRs and the lambda pieces the
-        node kernel sums in place of the runs' lambda entries."""
+        """Entries of the grouped gather's CSRs and the lambda value slots
+        the node kernel writes in place of the runs' lambda entries."""
         e, p = C.c_long(), C.c_long()
         check(lib().gdt_korobov_gather_info(self.h, C.byref(e), C.byref(p)))
-        return {"entries": e.value, "pieces": p.value}
+        return {"entries": e.value, "slots": p.value}
 
     def korobov_set_tiles(self, tiles_max):
         """Limit of gather tiles per node group (default 64); rebuilds the

@@ -1328,12 +1328,12 @@ int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap)
     return GDT_OK;
 }
 
-int gdt_korobov_gather_info(gdt_handle h, long *entries, long *pieces)
+int gdt_korobov_gather_info(gdt_handle h, long *entries, long *slots)
 {
     H_CHECK(h);
     GDT_CHECK(h->have_kor && h->have_stream, GDT_E_STATE, "gdt_korobov_gather_info: stencil not built");
     if (entries) *entries = h->h_gp.empty() ? 0 : h->h_gp.back();
-    if (pieces) *pieces = h->nv - h->nn;
+    if (slots) *slots = h->nv - h->nn;
     return GDT_OK;
 }
 

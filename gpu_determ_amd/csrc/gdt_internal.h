@@ -111,7 +111,7 @@ struct gdt_context {
     uint32_t kM = 0, ka = 0;
     double ks0 = 0.0, kK = 0.0;
     long nn = 0, nent = 0;
-    long nv = 0;  // node values per cell of the grouped path: nn nodes, then the lambda pieces
+    long nv = 0;  // node values per cell of the grouped path: nn nodes, then the lambda value slots
     DevBuf kidx, kr, kw, knode, kptr, kent;
     // per-node records (one uint4: six 15-bit node indices, num / 4, den / 4
     // and D2) and, for the fused path, per-entry records (uint4 as the
@@ -119,7 +119,7 @@ struct gdt_context {
     bool have_stream = false;
     DevBuf kn4, ke4, ke1, kwb;
     DevBuf kgptr, kgent;          // per node-group CSRs of the gather (uint2 entry records)
-    DevBuf kcbase;                // first lambda piece of each chunk of KOR_CHUNK node positions
+    DevBuf kcbase;                // first lambda value slot of each chunk of KOR_CHUNK node positions
     DevBuf ktile;                 // gather tiles (d0, d1, e0, e1), [group][kntile]
     DevBuf kmath;                 // log / exp tables of the node terms (KMT_SIZE doubles)
     int kntile = 0, kcap = 0;     // tiles per group, LDS entries per gather pass
@@ -275,8 +275,8 @@ struct KorClassDev {
     double *mp;    // [slots][20][nparts]
     int node_blocks;  // node work items per cell
     int pad;
-    long nv;               // node values per cell (nodes, then lambda pieces)
-    const int32_t *cbase;  // first lambda piece per chunk of node positions
+    long nv;               // node values per cell (nodes, then lambda value slots)
+    const int32_t *cbase;  // first lambda value slot per chunk of node positions
 };
 // Work item: (class, cell, block within the cell, slot)
 int launch_batch_node(hipStream_t s, const KorClassDev *cls, const int4 *items, int nitems, int blocks, int lds);

@@ -10,19 +10,20 @@
 // Default path, three kernels per step:
 //   k_kor_node    node-parallel, f of the cell staged in LDS (16-byte LDS
 //                 DMA): D and 2wP of every node, once, written contiguously
-//                 (double2 per node), and the sums of the lambda pieces (a
-//                 run of one lambda destination within a chunk of 16 node
-//                 positions; the records are sorted by lambda); block b
-//                 evaluates a slice of node group b % KOR_GROUPS, on the XCD
-//                 that gathers that group;
+//                 (double2 per node), and the lambda value slots (prefix
+//                 sums of the lambda terms over each chunk of 16 node
+//                 positions; the records are sorted by lambda, so a run of
+//                 one lambda within a chunk reads as two slots at most);
+//                 block b evaluates a slice of node group b % KOR_GROUPS, on
+//                 the XCD that gathers that group;
 //   k_kor_gather  destination-sorted reduction, XCD-aware: the nodes are
 //                 split into KOR_GROUPS contiguous ranges, each with its own
 //                 destination-sorted CSR of (node | role, coefficient -1 /
-//                 1-r / r) entries, where one entry per lambda piece (its
-//                 sum, coefficient 1) stands for the piece's lambda entries.
+//                 1-r / r) entries, where the value slots (coefficient 1)
+//                 stand for the lambda entries.
 //                 Workgroups are dealt to the XCDs round robin, so block b
 //                 serves group b % KOR_GROUPS and every XCD gathers only its
-//                 range's (D, 2wP) and piece sums -- under 1 MB, resident in
+//                 range's (D, 2wP) and slots -- under 1 MB, resident in
 //                 that XCD's L2.  A block takes one tile of a group's
 //                 destinations: it parks every entry's term in LDS, then a
 //                 thread (or, for tiles of long rows, a quad) sums each

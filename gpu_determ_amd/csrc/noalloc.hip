@@ -56,7 +56,7 @@ extern "C" int gdt_evaluate_noalloc(gdt_handle h, const double *f, double *nu, d
         base = h->device >= 0 && h->device < NOALLOC_MAX_DEVICES ? g_scratch_ptr[h->device] : nullptr;
     }
     GDT_CHECK(base, GDT_E_STATE, "gdt_evaluate_noalloc: no scratch address for device %d", h->device);
-    // node values (D, 2wP) per node and the lambda pieces, per cell; group
+    // node values (D, 2wP) per node and the lambda value slots, per cell; group
     // partials of Q and loss; moment sums per gather tile
     const int nparts = kor_gather_parts(h);
     const size_t vb = align256(sizeof(double) * 2 * (size_t)h->nv * cells);

@@ -138,12 +138,14 @@ int gdt_korobov_export(gdt_handle h, int32_t *idx6, double *r, double *w, uint32
  * tiles per group (one moment partial each), LDS entries per gather pass. */
 int gdt_korobov_tiles(gdt_handle h, int *groups, int *tiles, int *cap);
 /* Sizes of the grouped gather; no reference counterpart (layout figures for
- * byte counts).  The node kernel sums each lambda piece -- a run of one
- * lambda destination within a chunk of 16 node positions -- and the gather
- * reads the piece as one entry in place of the run's lambda entries:
- * *entries is the entries of the node groups' CSRs, *pieces the pieces (node
- * values per cell: nodes + pieces). */
-int gdt_korobov_gather_info(gdt_handle h, long *entries, long *pieces);
+ * byte counts).  The node kernel forms the prefix sums of the lambda terms
+ * over each chunk of 16 node positions and writes each lambda piece (a run
+ * of one lambda destination within a chunk) as value slots: +prefix at its
+ * last position and -prefix before its first, unless it starts the chunk.
+ * The gather reads each slot as one entry in place of the piece's lambda
+ * entries.  *entries is the entries of the node groups' CSRs, *slots the
+ * value slots (node values per cell: nodes + slots). */
+int gdt_korobov_gather_info(gdt_handle h, long *entries, long *slots);
 /* Limit of gather tiles per node group (default 64, 1..1024); no reference
  * counterpart (a launch-shape setting of the gather).  A tile holds at most
  * 512 destinations, so a group has at least ceil(grid nodes / 512) tiles

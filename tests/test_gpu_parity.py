@@ -247,7 +247,7 @@ def test_korobov_bits_do_not_depend_on_cells_per_call():
     c = ctx_for(g)
     c.korobov_build(1 << 16, seed=5, s0=0.25)
     gi = c.korobov_gather_info()
-    assert 0 < gi["pieces"] < c.korobov_info()["nodes"]
+    assert 0 < gi["slots"] < c.korobov_info()["nodes"]
     rng = np.random.default_rng(21)
     f = np.stack([mixture(g.xi, rng) for _ in range(300)])
     fd = G.DeviceArray.from_host(f)
@@ -261,6 +261,29 @@ def test_korobov_bits_do_not_depend_on_cells_per_call():
         assert np.array_equal(l.get(), L300[:cells])
         assert np.array_equal(q.get(), Q300[:cells])
         assert np.array_equal(qc.get(), Qc300[:cells])
+
+
+@pytest.mark.parametrize("n,M", [(12, 1 << 10), (16, 1 << 12)])
+def test_korobov_small_stencils(n, M):
+    """Stencils of a few hundred nodes (190 at 12^3 with 2^10 lattice nodes):
+    node groups of 16 to 32 positions, most node blocks empty, partial waves
+    and chunks of lambda value slots at the group ends.  One cell and 40 cells
+    against the oracle, and the same bits either way."""
+    say("korobov small stencil", n, M)
+    vm = 4.8
+    g = O.Grid((n,) * 3, (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    c.korobov_build(M, seed=3, s0=0.25)
+    st = O.korobov_stencil(g, M, c.korobov_info()["a"], 0.25)
+    say("  nodes", c.korobov_info()["nodes"], c.korobov_gather_info())
+    rng = np.random.default_rng(n)
+    f = np.stack([mixture(g.xi, rng) for _ in range(40)])
+    loss, Q = c.collide_korobov(G.DeviceArray.from_host(f), 40)
+    ol, oq = O.korobov_collide(g, st, f)
+    assert rel_l2(loss.get(), ol) < 1e-12
+    assert rel_l2(Q.get(), oq) < 1e-12
+    l1, q1 = c.collide_korobov(G.DeviceArray.from_host(f[:1]), 1)
+    assert np.array_equal(q1.get(), Q.get()[:1]) and np.array_equal(l1.get(), loss.get()[:1])
 
 
 class _DeviceView:
