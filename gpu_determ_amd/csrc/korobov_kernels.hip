@@ -130,7 +130,7 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, 
 // ---------------------------------------------------------------------------
 // Per node t of cell c, the values the destination entries read:
 //   V[t] = (D, 2 w P)
-// and, after the nn nodes, one value per lambda piece (below).
+// and, after the nn nodes, the lambda value slots (below).
 // Block b of a grid of B blocks per cell owns a contiguous node range:
 //   B a multiple of KOR_GROUPS (xcd): the range is slice b / G of the S = B / G
 //     equal slices of gather group g = b % G.  Workgroups are dealt to the
@@ -138,20 +138,20 @@ __device__ __forceinline__ void stage_f(double *fs, const double *fc, int npts, 
 //     whose L2 the gather of group g then reads them through (one step:
 //     29.0 -> 26.7 us against group-blind ranges);
 //   otherwise: [b per, (b + 1) per), per = ceil(nn / B).
-// Slices are whole waves of positions (groups start on chunk boundaries),
-// so every block has the same share within a wave and lane i of a wave
-// holds a position i mod 64.  A thread owns nodes t and t + KOR_THREADS of
-// the range and evaluates both together, so the two dependency chains (LDS
-// reads, log / exp) overlap; their records are loaded before the staging of
-// f, so that latency overlaps too.  Further nodes (more than two per thread)
-// are handled one at a time.  f of the cell is fc; values go to out (the
-// cell's V).
+// Slices are whole waves of positions (per rounded up to 64) and groups
+// start on chunk boundaries, so every block has the same share within a
+// wave, and a wave's 64 positions start on a chunk boundary.  A thread owns
+// nodes t and t + KOR_THREADS of the range and evaluates both together, so
+// the two dependency chains (LDS reads, log / exp) overlap; their records
+// are loaded before the staging of f, so that latency overlaps too.  Further
+// nodes (more than two per thread) are handled one at a time.  f of the cell
+// is fc; values go to out (the cell's V).
 //
 // Lambda pieces.  The records are sorted by the lambda destination, so a
 // destination's lambda terms (1 - r) D come from one run of positions.  The
 // positions are cut into chunks of KOR_CHUNK = 16, one DPP row of a wave
-// (lane i of a wave holds position 64 c + i), and a piece is a run of one
-// lambda within a chunk.  The row forms the prefix sums I of its terms
+// (a wave holds 64 consecutive positions from a chunk boundary on), and a
+// piece is a run of one lambda within a chunk.  The row forms the prefix sums I of its terms
 // (row shifts 1, 2, 4, 8: a fixed pairing order) and a piece [s, e] is
 // I_e - I_(s-1): its last position writes +I_e and, unless it starts the
 // chunk, its first position writes -I_(s-1), each to its own value slot
