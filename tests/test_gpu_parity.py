@@ -208,6 +208,31 @@ def test_korobov_stencil_bitexact(n, M, s0):
         assert np.all(np.diff(seg) >= 0)
 
 
+def test_korobov_explicit_generator():
+    """A caller's own odd generator a (no figure-of-merit pick) and s0 = 1:
+    the stencil is bit-exact against the oracle's for that a, and Q and the
+    loss match the oracle to 1e-12."""
+    say("korobov explicit generator")
+    vm = 4.8
+    g = O.Grid((16, 16, 16), (-vm,) * 3, (vm,) * 3, vm)
+    c = ctx_for(g)
+    M, a = 1 << 16, 20021
+    c.korobov_build(M, seed=0, a=a, s0=1.0)
+    info = c.korobov_info()
+    assert info["a"] == a
+    assert abs(info["K"] - O.korobov_K(g, M, 1.0)) <= 1e-15 * info["K"]
+    st = O.korobov_stencil(g, M, a, 1.0)
+    idx6, r, w, node = c.korobov_export()
+    assert np.array_equal(idx6.reshape(-1, 6), st[0]) and np.array_equal(r, st[1])
+    assert np.array_equal(w, st[2]) and np.array_equal(node, st[3])
+    rng = np.random.default_rng(31)
+    f = np.stack([mixture(g.xi, rng) for _ in range(3)])
+    loss, Q = c.collide_korobov(G.DeviceArray.from_host(f), 3)
+    ol, oq = O.korobov_collide(g, st, f)
+    assert rel_l2(loss.get(), ol) < 1e-12
+    assert rel_l2(Q.get(), oq) < 1e-12
+
+
 # 1 and 3 cells: node-kernel ranges follow the gather groups (256 and 88
 # blocks per cell); 40 cells: 7 blocks per cell, plain contiguous ranges
 @pytest.mark.parametrize("cells", [1, 3, 40])
