@@ -14,6 +14,7 @@
   launch limit of 65,535 cells are refused with an error; bad arguments
   return the ABI's error codes.
 """
+import ctypes as C
 import sys
 
 import numpy as np
@@ -278,6 +279,8 @@ def test_argument_errors():
     assert ev(G.METHOD_KOROBOV, f.ptr, 1) == E_STATE  # stencil not built
     assert ev(G.METHOD_MATRIX, f.ptr, 1) == E_STATE   # tables not built
     assert "not built" in L.gdt_last_error().decode()
+    ne, ns = C.c_long(), C.c_long()
+    assert L.gdt_korobov_gather_info(c.h, C.byref(ne), C.byref(ns)) == E_STATE  # stencil not built
     assert L.gdt_korobov_build(c.h, 3000, 1, 0, 0.25) == E_ARG      # M not a power of two
     assert L.gdt_korobov_build(c.h, 1 << 12, 1, 6, 0.25) == E_ARG   # even generator
     assert L.gdt_korobov_build(c.h, 1 << 12, 1, 0, -1.0) == E_ARG   # s0 <= 0
